@@ -1,0 +1,227 @@
+"""ctypes binding to liboracle.so (test infrastructure; see package docstring)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build_oracle(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile (g++/OpenMP)."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        build_oracle()
+    L = C.CDLL(_LIB)
+    vp = C.c_void_p
+    L.orc_create.restype = vp
+    L.orc_create.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                             C.c_int, C.c_void_p, C.c_void_p]
+    L.orc_destroy.argtypes = [vp]
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_last_error.argtypes = [vp]
+    L.orc_ok.argtypes = [vp]
+    L.orc_warnings.argtypes = [vp]
+    L.orc_info.argtypes = [vp, _ip]
+    L.orc_get.restype = C.c_longlong
+    L.orc_get.argtypes = [vp, C.c_char_p, C.c_int, C.c_void_p, C.c_longlong]
+    L.orc_set_modulus.argtypes = [vp, _dp]
+    L.orc_assemble.argtypes = [vp]
+    L.orc_apply_K.argtypes = [vp, _dp, _dp]
+    L.orc_vcycle.argtypes = [vp, _dp, _dp]
+    L.orc_coarse_solve.argtypes = [vp, _dp, _dp]
+    L.orc_restrict.argtypes = [vp, C.c_int, _dp, _dp]
+    L.orc_interpolate.argtypes = [vp, C.c_int, _dp, _dp]
+    L.orc_pcg.argtypes = [vp, _dp, _dp, C.c_double, C.c_int, C.POINTER(C.c_double)]
+    L.orc_compliance.argtypes = [vp, _dp, _dp]
+    L.orc_filter_setup.argtypes = [vp, C.c_int, C.c_double]
+    L.orc_filter_apply.argtypes = [vp, C.c_int, _dp, _dp]
+    L.orc_filter.argtypes = [vp, C.c_int, _dp, _dp]
+    L.orc_oc.restype = C.c_double
+    L.orc_oc.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_int)]
+    L.orc_mma.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
+    L.orc_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                          _dp, _dp, _dp, _dp, _dp, _ip, _dp, C.c_void_p, C.c_int,
+                          C.POINTER(C.c_double), C.POINTER(C.c_int), _ip]
+    L.orc_num_threads.restype = C.c_int
+    _lib = L
+    return L
+
+
+def cuboid_bc_oracle(nely: int, nelx: int, nelz: int):
+    """Marker: use the reference's built-in cuboid BCs (TOP3D_XL.m:536-559)."""
+    return None
+
+
+class Oracle:
+    """One oracle model (TOP3D_XL.m state after CreateVoxelFEAmodel + setup)."""
+
+    def __init__(self, voxels: np.ndarray, fix=None, load=None):
+        L = _load()
+        vox = np.asfortranarray(voxels.astype(np.uint8))
+        nely, nelx, nelz = vox.shape
+        flat = np.ascontiguousarray(vox.reshape(-1, order="F"))
+        self._keep = [flat]
+        if fix is None:
+            h = L.orc_create(flat, nely, nelx, nelz, -1, None, None, 0, None, None)
+        else:
+            fn, fs = fix
+            ln, lv = load
+            fn = np.ascontiguousarray(fn, dtype=np.int32)
+            fs = np.ascontiguousarray(fs, dtype=np.uint8).reshape(-1)
+            ln = np.ascontiguousarray(ln, dtype=np.int32)
+            lv = np.ascontiguousarray(lv, dtype=np.float64).reshape(-1)
+            self._keep += [fn, fs, ln, lv]
+            h = L.orc_create(flat, nely, nelx, nelz, len(fn), fn.ctypes.data, fs.ctypes.data,
+                             len(ln), ln.ctypes.data, lv.ctypes.data)
+        self.h = h
+        self.L = L
+        if not L.orc_ok(h):
+            msg = L.orc_last_error(h).decode()
+            L.orc_destroy(h)
+            self.h = None
+            raise ValueError(msg)
+        info = np.zeros(1 + 8 * 16, dtype=np.int32)
+        L.orc_info(h, info)
+        self.num_levels = int(info[0])
+        self.levels = []
+        for l in range(self.num_levels):
+            o = info[1 + 8 * l: 9 + 8 * l]
+            self.levels.append(dict(resX=int(o[0]), resY=int(o[1]), resZ=int(o[2]), span=int(o[3]),
+                                    numElements=int(o[4]), numNodes=int(o[5]),
+                                    intermediateNumNodes=int(o[6])))
+        self.nE = self.levels[0]["numElements"]
+        self.nN = self.levels[0]["numNodes"]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    # ------------------------------------------------------------- accessors
+    _types = {"transferMatCoeffi": np.float64, "diagK": np.float64, "Ks": np.float64, "RI": np.float64,
+              "Ke": np.float64, "F": np.float64, "KesFree": np.float64, "KesFixed": np.float64,
+              "filterKernel0": np.float64, "filterDiag0": np.float64, "fixState": np.uint8}
+
+    def get(self, name: str, level: int = 0) -> np.ndarray:
+        n = self.L.orc_get(self.h, name.encode(), level, None, 0)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(int(n), dtype=self._types.get(name, np.int32))
+        self.L.orc_get(self.h, name.encode(), level, out.ctypes.data, int(n))
+        return out
+
+    # --------------------------------------------------------------- steps
+    def set_modulus(self, E):
+        self.L.orc_set_modulus(self.h, np.ascontiguousarray(E, dtype=np.float64))
+
+    def assemble(self):
+        if self.L.orc_assemble(self.h) != 0:
+            raise RuntimeError(self.L.orc_last_error(self.h).decode())
+
+    def apply_K(self, u):
+        y = np.zeros(3 * self.nN)
+        self.L.orc_apply_K(self.h, np.ascontiguousarray(u, dtype=np.float64), y)
+        return y
+
+    def vcycle(self, r):
+        x = np.zeros(3 * self.nN)
+        self.L.orc_vcycle(self.h, np.ascontiguousarray(r, dtype=np.float64), x)
+        return x
+
+    def coarse_solve(self, r):
+        x = np.zeros_like(r, dtype=np.float64)
+        self.L.orc_coarse_solve(self.h, np.ascontiguousarray(r, dtype=np.float64), x)
+        return x
+
+    def restrict(self, level, rF):
+        rC = np.zeros(3 * self.levels[level]["numNodes"])
+        self.L.orc_restrict(self.h, level, np.ascontiguousarray(rF, dtype=np.float64), rC)
+        return rC
+
+    def interpolate(self, level, xC):
+        xF = np.zeros(3 * self.levels[level - 1]["numNodes"])
+        self.L.orc_interpolate(self.h, level, np.ascontiguousarray(xC, dtype=np.float64), xF)
+        return xF
+
+    def pcg(self, b, x0=None, tol=1e-3, maxit=800):
+        x = np.zeros(3 * self.nN) if x0 is None else np.array(x0, dtype=np.float64)
+        res = C.c_double(0.0)
+        its = self.L.orc_pcg(self.h, np.ascontiguousarray(b, dtype=np.float64), x, tol, maxit, C.byref(res))
+        return x, its, res.value
+
+    def compliance(self, U):
+        ce = np.zeros(self.nE)
+        self.L.orc_compliance(self.h, np.ascontiguousarray(U, dtype=np.float64), ce)
+        return ce
+
+    def filter_setup(self, slot, r):
+        self.L.orc_filter_setup(self.h, slot, float(r))
+
+    def filter_apply(self, slot, u):
+        y = np.zeros(self.nN)
+        self.L.orc_filter_apply(self.h, slot, np.ascontiguousarray(u, dtype=np.float64), y)
+        return y
+
+    def filter(self, slot, src):
+        dst = np.zeros(self.nE)
+        its = self.L.orc_filter(self.h, slot, np.ascontiguousarray(src, dtype=np.float64), dst)
+        return dst, its
+
+    def oc(self, x, xPhys, dc, V0):
+        xx = np.array(x, dtype=np.float64)
+        nb = C.c_int(0)
+        ch = self.L.orc_oc(self.h, xx, np.ascontiguousarray(xPhys, dtype=np.float64),
+                           np.ascontiguousarray(dc, dtype=np.float64), float(V0), C.byref(nb))
+        return xx, ch, nb.value
+
+    @staticmethod
+    def mma(iter_, xval, xmin, xmax, xold1, xold2, dfdx, gx, dgdx, low, upp):
+        L = _load()
+        n = len(xval)
+        arrs = [np.array(a, dtype=np.float64) for a in (xval, xmin, xmax, xold1, xold2, dfdx, dgdx, low, upp)]
+        xv, xmn, xmx, xo1, xo2, df, dg, lo, up = arrs
+        nn = L.orc_mma(n, iter_, xv, xmn, xmx, xo1, xo2, df, float(gx), dg, lo, up)
+        return dict(x=xv, xold1=xo1, xold2=xo2, low=lo, upp=up, newton=nn)
+
+    def run(self, mode, V0, nLoop, rMin, rHat=0.0, nSnap=0):
+        nE = self.nE
+        xPhys = np.zeros(nE)
+        cH, vH, conH, sH = (np.zeros(max(nLoop, 1)) for _ in range(4))
+        its = np.zeros(max(nLoop, 1), dtype=np.int32)
+        fits = np.zeros(max(nLoop, 1), dtype=np.int32)
+        tH = np.zeros(max(nLoop, 1) * 6)
+        snap = np.zeros(max(nSnap, 1) * nE)
+        cS = C.c_double(0.0)
+        loops = C.c_int(0)
+        m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
+        rc = self.L.orc_run(self.h, m, float(V0), int(nLoop), float(rMin), float(rHat), xPhys, cH, vH, conH, sH,
+                            its, tH, snap.ctypes.data, int(nSnap), C.byref(cS), C.byref(loops), fits)
+        if rc != 0:
+            raise RuntimeError(self.L.orc_last_error(self.h).decode())
+        k = loops.value
+        return dict(xPhys=xPhys, cHist=cH[:k], volHist=vH[:k], consHist=conH[:k], sharpHist=sH[:k],
+                    lssIts=its[:k], filterIts=fits[:k], tHist=tH[:6 * k].reshape(k, 6),
+                    xSnap=snap[:min(nSnap, k) * nE].reshape(-1, nE), cSolid=cS.value, loops=k,
+                    warnings=self.L.orc_warnings(self.h))
+
+    @staticmethod
+    def num_threads():
+        return _load().orc_num_threads()
