@@ -59,7 +59,7 @@ def _load():
     L.orc_mma.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
     L.orc_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
                           _dp, _dp, _dp, _dp, _dp, _ip, _dp, C.c_void_p, C.c_int,
-                          C.POINTER(C.c_double), C.POINTER(C.c_int), _ip]
+                          C.POINTER(C.c_double), C.POINTER(C.c_int), _ip, _dp]
     L.orc_num_threads.restype = C.c_int
     _lib = L
     return L
@@ -207,18 +207,19 @@ class Oracle:
         cH, vH, conH, sH = (np.zeros(max(nLoop, 1)) for _ in range(4))
         its = np.zeros(max(nLoop, 1), dtype=np.int32)
         fits = np.zeros(max(nLoop, 1), dtype=np.int32)
+        resH = np.zeros(max(nLoop, 1))
         tH = np.zeros(max(nLoop, 1) * 6)
         snap = np.zeros(max(nSnap, 1) * nE)
         cS = C.c_double(0.0)
         loops = C.c_int(0)
         m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
         rc = self.L.orc_run(self.h, m, float(V0), int(nLoop), float(rMin), float(rHat), xPhys, cH, vH, conH, sH,
-                            its, tH, snap.ctypes.data, int(nSnap), C.byref(cS), C.byref(loops), fits)
+                            its, tH, snap.ctypes.data, int(nSnap), C.byref(cS), C.byref(loops), fits, resH)
         if rc != 0:
             raise RuntimeError(self.L.orc_last_error(self.h).decode())
         k = loops.value
         return dict(xPhys=xPhys, cHist=cH[:k], volHist=vH[:k], consHist=conH[:k], sharpHist=sH[:k],
-                    lssIts=its[:k], filterIts=fits[:k], tHist=tH[:6 * k].reshape(k, 6),
+                    lssIts=its[:k], filterIts=fits[:k], resHist=resH[:k], tHist=tH[:6 * k].reshape(k, 6),
                     xSnap=snap[:min(nSnap, k) * nE].reshape(-1, nE), cSolid=cS.value, loops=k,
                     warnings=self.L.orc_warnings(self.h))
 

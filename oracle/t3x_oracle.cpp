@@ -1289,6 +1289,7 @@ struct RunOut {
   double* cSolid = nullptr;
   int* loops = nullptr;
   int* filterIts = nullptr;    // nLoop (sum of filter CG iterations)
+  double* resHist = nullptr;   // nLoop: final relative residual of each elasticity solve
 };
 
 inline double pow16(double v) { double v2 = v * v, v4 = v2 * v2, v8 = v4 * v4; return v8 * v8; }
@@ -1325,7 +1326,8 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
     const double ta = now_s();
     if (!assemble(M)) return -1;  // 118
     const double tb = now_s();
-    int its = solve_fea(M, U, &warned, nullptr);  // 121
+    double resFinal = 0.0;
+    int its = solve_fea(M, U, &warned, &resFinal);  // 121
     if (warned) M.warnings++;
     const double tc = now_s();
     compliance(M, U.data(), ce.data());  // 124
@@ -1398,6 +1400,7 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
     if (O.sharpHist) O.sharpHist[k] = sharp;
     if (O.lssIts) O.lssIts[k] = its;
     if (O.filterIts) O.filterIts[k] = fIts;
+    if (O.resHist) O.resHist[k] = resFinal;
     if (O.tHist) {
       double* th = &O.tHist[(size_t)k * 6];
       th[0] = tb - ta; th[1] = tc - tb; th[2] = tOpt; th[3] = tFilt; th[4] = tLVF; th[5] = tTot;
@@ -1579,13 +1582,14 @@ int orc_mma(int n, int iter, double* xval, const double* xmin, const double* xma
 // Whole optimisation (mode 0 = GLOBAL / TOP3D_XL_TO, 1 = LOCAL / TOP3D_XL_PIO).
 int orc_run(void* h, int mode, double V0, int nLoop, double rMin, double rHat, double* xPhys, double* cHist,
             double* volHist, double* consHist, double* sharpHist, int* lssIts, double* tHist, double* xSnap,
-            int nSnap, double* cSolid, int* loops, int* filterIts) {
+            int nSnap, double* cSolid, int* loops, int* filterIts, double* resHist) {
   Model* M = (Model*)h;
   if (!M->bcOk) return -1;
   RunOut O;
   O.xPhys = xPhys; O.cHist = cHist; O.volHist = volHist; O.consHist = consHist; O.sharpHist = sharpHist;
   O.lssIts = lssIts; O.tHist = tHist; O.xSnap = xSnap; O.nSnap = nSnap; O.cSolid = cSolid; O.loops = loops;
   O.filterIts = filterIts;
+  O.resHist = resHist;
   return run(*M, mode, V0, nLoop, rMin, rHat, O);
 }
 

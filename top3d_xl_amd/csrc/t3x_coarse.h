@@ -1,0 +1,27 @@
+// Coarsest-level banded Cholesky factor + block-bidiagonal solves (see t3x_coarse.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace t3x {
+
+struct CoarseSolver {
+  int n = 0;     // true order
+  int bw = 0;    // bandwidth
+  int SB = 0;    // solve block (multiple of 64, >= bw + 1)
+  int np = 0;    // padded order (multiple of SB)
+  int nblk = 0;  // np / SB
+  void plan(int n, int bw);
+  size_t band_len() const;
+  size_t tinv_len() const { return (size_t)np * 64; }
+  size_t blk_len() const { return (size_t)nblk * SB * SB; }
+  void pad_identity(hipStream_t s, double* band);
+  void factor(hipStream_t s, double* band, double* tinv, double* Dinv, double* DT, double* Gm, double* H,
+              int* status);
+  void solve(hipStream_t s, const long long* map, const double* rdense, double* xdense, const double* Dinv,
+             const double* DT, const double* Gm, const double* H, double* va, double* vb, const int* fl);
+};
+
+}  // namespace t3x

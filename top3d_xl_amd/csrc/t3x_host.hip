@@ -1,0 +1,1193 @@
+// Host side of libt3x: model set-up, the per-iteration drivers and the C ABI.
+//
+// Reference lines are TOP3D_XL.m.  There is no CPU compute path: every
+// numerical step runs in the HIP kernels of t3x_kernels.hip; a missing GPU is
+// an error (T3X_ERR_NO_DEVICE).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/t3x.h"
+#include "t3x_coarse.h"
+#include "t3x_kernels.h"
+#include "t3x_reduce.h"
+
+using namespace t3x;
+
+namespace {
+
+double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  T3X_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  T3X_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  return (T*)p;
+}
+
+// ------------------------------------------------------------ element stiffness
+// Ke of the unit trilinear brick for E = 1 from 2x2x2 Gauss quadrature of
+// B^T D B (exact for a brick), nodes in the reference's local order at
+// physical (x, y, z) = (dx, -dy, dz).  Equals the closed form of
+// FEA_VoxelBasedElementStiffnessMatrix (1254-1292) to rounding.
+std::vector<double> gauss_ke(double nu, double cell) {
+  const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1.0 / (2 * (1 + nu));
+  double D[6][6] = {};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D[i][j] = lam;
+  for (int i = 0; i < 3; ++i) D[i][i] += 2 * mu;
+  for (int i = 3; i < 6; ++i) D[i][i] = mu;
+  double nat[8][3];
+  for (int a = 0; a < 8; ++a) {
+    nat[a][0] = kLocDx[a] ? 1.0 : -1.0;   // x
+    nat[a][1] = kLocDy[a] ? -1.0 : 1.0;   // y = -row
+    nat[a][2] = kLocDz[a] ? 1.0 : -1.0;   // z
+  }
+  std::vector<double> K(576, 0.0);
+  const double g = 1.0 / std::sqrt(3.0);
+  for (int q = 0; q < 8; ++q) {
+    const double xi = (q & 1) ? g : -g, eta = (q & 2) ? g : -g, zeta = (q & 4) ? g : -g;
+    double B[6][24] = {};
+    for (int a = 0; a < 8; ++a) {
+      const double s = nat[a][0], t = nat[a][1], p = nat[a][2];
+      // d/dx = 2 d/dxi on the unit cube
+      const double bx = 2 * 0.125 * s * (1 + t * eta) * (1 + p * zeta);
+      const double by = 2 * 0.125 * t * (1 + s * xi) * (1 + p * zeta);
+      const double bz = 2 * 0.125 * p * (1 + s * xi) * (1 + t * eta);
+      B[0][3 * a] = bx; B[1][3 * a + 1] = by; B[2][3 * a + 2] = bz;
+      B[3][3 * a + 1] = bz; B[3][3 * a + 2] = by;
+      B[4][3 * a] = bz; B[4][3 * a + 2] = bx;
+      B[5][3 * a] = by; B[5][3 * a + 1] = bx;
+    }
+    const double w = 0.125;  // det J
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 6; ++k)
+          for (int l = 0; l < 6; ++l) s += B[k][i] * D[k][l] * B[l][j];
+        K[j * 24 + i] += s * w;
+      }
+  }
+  for (int i = 0; i < 24; ++i)  // exact symmetry
+    for (int j = 0; j < i; ++j) {
+      const double v = 0.5 * (K[j * 24 + i] + K[i * 24 + j]);
+      K[j * 24 + i] = K[i * 24 + j] = v * cell;
+    }
+  for (int i = 0; i < 24; ++i) K[i * 25] *= cell;
+  return K;
+}
+
+// Interpolation weight of coarse local node J at child-local node a of child
+// (ly,lx,lz) of a span-s cell.
+double hat_weight(int span, int ly, int lx, int lz, int a, int J) {
+  const int ty = ly + kLocDy[a], tx = lx + kLocDx[a], tz = lz + kLocDz[a];
+  const double wy = kLocDy[J] ? (double)ty / span : (double)(span - ty) / span;
+  const double wx = kLocDx[J] ? (double)tx / span : (double)(span - tx) / span;
+  const double wz = kLocDz[J] ? (double)tz / span : (double)(span - tz) / span;
+  return wy * wx * wz;
+}
+
+// T_ss = Q_ss^T Ke Q_ss (upper triangle) for the span^3 children (950-954).
+std::vector<double> l2_templates(const std::vector<double>& Ke, int span) {
+  const int s3 = span * span * span;
+  std::vector<double> T((size_t)s3 * kTri);
+  for (int ss = 0; ss < s3; ++ss) {
+    const int ly = ss % span, lx = (ss / span) % span, lz = ss / (span * span);
+    double Q[24][24] = {};
+    for (int a = 0; a < 8; ++a)
+      for (int J = 0; J < 8; ++J) {
+        const double w = hat_weight(span, ly, lx, lz, a, J);
+        for (int c = 0; c < 3; ++c) Q[3 * a + c][3 * J + c] = w;
+      }
+    double KQ[24][24];
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 24; ++k) s += Ke[k * 24 + i] * Q[k][j];
+        KQ[i][j] = s;
+      }
+    for (int q = 0; q < 24; ++q)
+      for (int p = 0; p <= q; ++p) {
+        double s = 0.0;
+        for (int k = 0; k < 24; ++k) s += Q[k][p] * KQ[k][q];
+        T[(size_t)ss * kTri + tri_index(p, q)] = s;
+      }
+  }
+  return T;
+}
+
+}  // namespace
+
+// ============================================================== the problem
+struct OptState {
+  bool active = false;
+  int mode = 0, nLoop = 0, loop = 0, loopbeta = 0;
+  double V0 = 0.0, rMin = 0.0, rHat = 0.0, cSolid = 0.0;
+  double change = 1.0, sharp = 1.0, beta = 1.0;  // 85-86, 233, 284-285
+  std::vector<t3x_iter> hist;
+};
+
+// HIP-event timing of kernel groups on the problem's stream.
+enum TimedGroup { TG_APPLYK = 0, TG_VCYCLE, TG_COARSE, TG_ASSEMBLE, TG_COUNT };
+struct GroupTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<hipEvent_t> pool;
+  double ms = 0.0;
+  long long count = 0;
+};
+
+struct t3x_problem {
+  OptState opt;
+  GroupTimer timers[TG_COUNT];
+  t3x_params P{};
+  int nely = 0, nelx = 0, nelz = 0;
+  int L = 0;
+  std::vector<Level> lv;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int warnings = 0;
+  // finest-level device state
+  double* dKe = nullptr;
+  double* dT2 = nullptr;
+  uint8_t* fix = nullptr;
+  uint8_t* passive = nullptr;
+  double* bvec = nullptr;
+  double *F = nullptr, *U = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp3 = nullptr;
+  double *E = nullptr, *x = nullptr, *xPhys = nullptr, *ce = nullptr, *dc = nullptr;
+  double *ca = nullptr, *cb = nullptr, *cc = nullptr, *cd = nullptr;  // cell scratch
+  // filters (node scalars)
+  double *fb = nullptr, *fx = nullptr, *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr;
+  Filter filt[2];
+  double* dFiltKern = nullptr;  // 2*64
+  // L2 cells with a BC child
+  int nBc2 = 0;
+  int* dBc2 = nullptr;
+  uint8_t* dBcFlag2 = nullptr;
+  // coarsest band
+  int cN = 0, cBW = 0;
+  CoarseSolver cs;
+  long long* dCMap = nullptr;
+  int* dNodePos = nullptr;
+  double *band = nullptr, *tinv = nullptr, *cDinv = nullptr, *cDT = nullptr, *cG = nullptr, *cH = nullptr,
+         *cva = nullptr, *cvb = nullptr;
+  int* dStatus = nullptr;
+  bool assembled = false;
+  // compact <-> dense maps (finest)
+  int nNodes = 0, nCells = 0;
+  int *dNodeMap = nullptr, *dCellMap = nullptr;
+  std::vector<int> hNodeMap, hCellMap;
+  // reductions / scalars
+  RedWs ws{nullptr, nullptr};
+  double* sc = nullptr;   // [0..15] elastic PCG, [16..31] filter PCG, [32..] misc
+  int* fl = nullptr;      // [0..3] elastic, [4..7] filter
+  double* oc = nullptr;   // 8
+  double* mm = nullptr;   // MM_COUNT
+  double* hbuf = nullptr; // pinned read-back
+  // MMA state (dense cells)
+  double *mx = nullptr, *mxold1 = nullptr, *mxold2 = nullptr, *mlow = nullptr, *mupp = nullptr, *malpha = nullptr,
+         *mbeta = nullptr, *mp0 = nullptr, *mq0 = nullptr, *mpij = nullptr, *mqij = nullptr, *mxmin = nullptr,
+         *mxmax = nullptr;
+  std::vector<void*> allocs;
+
+  template <class T>
+  T* alloc(size_t n) {
+    T* p = dalloc<T>(n);
+    allocs.push_back(p);
+    return p;
+  }
+  hipEvent_t ev() {
+    hipEvent_t e;
+    T3X_HIP(hipEventCreate(&e));
+    return e;
+  }
+  // begin/end of a timed launch group (no-op unless params.timing)
+  void tbegin(int g, hipEvent_t* e) {
+    if (!P.timing) return;
+    GroupTimer& T = timers[g];
+    if (T.pool.size() < 2) { T.pool.push_back(ev()); T.pool.push_back(ev()); }
+    e[0] = T.pool.back(); T.pool.pop_back();
+    e[1] = T.pool.back(); T.pool.pop_back();
+    T3X_HIP(hipEventRecord(e[0], stream));
+  }
+  void tend(int g, hipEvent_t* e) {
+    if (!P.timing) return;
+    GroupTimer& T = timers[g];
+    T3X_HIP(hipEventRecord(e[1], stream));
+    T.pending.push_back({e[0], e[1]});
+    if (T.pending.size() > 4096) tflush(g);
+  }
+  void tflush(int g) {
+    GroupTimer& T = timers[g];
+    for (auto& pr : T.pending) {
+      T3X_HIP(hipEventSynchronize(pr.second));
+      float ms = 0.f;
+      T3X_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      T.ms += ms;
+      T.count += 1;
+      T.pool.push_back(pr.first);
+      T.pool.push_back(pr.second);
+    }
+    T.pending.clear();
+  }
+  ~t3x_problem() {
+    for (auto& T : timers) {
+      for (auto& pr : T.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+      for (auto e : T.pool) (void)hipEventDestroy(e);
+    }
+    for (void* p : allocs) (void)hipFree(p);
+    if (hbuf) (void)hipHostFree(hbuf);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  const Level& f() const { return lv[0]; }
+  double* xTilde() { return ca; }   // LOCAL-mode state between iterations
+  double* volMax() { return cb; }
+  long long NN() const { return lv[0].NN; }
+  void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
+};
+
+namespace {
+
+// --------------------------------------------------------- hierarchy set-up
+// Level count and padding (1651-1671), non-dyadic merge (1085-1096).
+void build_levels(t3x_problem& P, const uint8_t* vox) {
+  long long nv = 0;
+  const long long nvox = (long long)P.nely * P.nelx * P.nelz;
+  for (long long i = 0; i < nvox; ++i) nv += vox[i] ? 1 : 0;
+  int cnt = 0;
+  while (nv >= P.P.coarsestControl) {
+    ++cnt;
+    nv = (long long)std::llround((double)nv / 8.0);
+  }
+  cnt = std::max(3, cnt);
+  const int pw = 1 << cnt;
+  auto padded = [&](int n) { return (n + pw - 1) / pw * pw; };
+  P.L = cnt;  // (cnt + 1) levels, then one merged by the span-4 first coarsening
+  P.lv.assign(P.L, Level());
+  Level& F = P.lv[0];
+  F.ny = padded(P.nely); F.nx = padded(P.nelx); F.nz = padded(P.nelz);
+  for (int l = 1; l < P.L; ++l) {
+    Level& C = P.lv[l];
+    const Level& Fn = P.lv[l - 1];
+    C.span = (l == 1) ? 4 : 2;
+    C.ny = Fn.ny / C.span; C.nx = Fn.nx / C.span; C.nz = Fn.nz / C.span;
+  }
+  for (int l = 0; l < P.L; ++l) {
+    Level& V = P.lv[l];
+    V.NY = V.ny + 1; V.NX = V.nx + 1; V.NZ = V.nz + 1;
+    V.NN = (long long)V.NY * V.NX * V.NZ;
+    V.hEmask.assign(V.NN, 0);
+    V.hNmask.assign(V.NN, 0);
+  }
+  // finest cells
+  for (int iz = 0; iz < P.nelz; ++iz)
+    for (int ix = 0; ix < P.nelx; ++ix)
+      for (int iy = 0; iy < P.nely; ++iy)
+        if (vox[iy + (long long)P.nely * (ix + (long long)P.nelx * iz)])
+          F.hEmask[iy + (long long)F.NY * (ix + (long long)F.NX * iz)] = 1;
+  // coarse cells: non-empty if any child is (1152)
+  for (int l = 1; l < P.L; ++l) {
+    Level& C = P.lv[l];
+    const Level& Fn = P.lv[l - 1];
+    const int s = C.span;
+    for (int iz = 0; iz < Fn.nz; ++iz)
+      for (int ix = 0; ix < Fn.nx; ++ix)
+        for (int iy = 0; iy < Fn.ny; ++iy)
+          if (Fn.hEmask[iy + (long long)Fn.NY * (ix + (long long)Fn.NX * iz)])
+            C.hEmask[iy / s + (long long)C.NY * (ix / s + (long long)C.NX * (iz / s))] = 1;
+  }
+  for (int l = 0; l < P.L; ++l) {
+    Level& V = P.lv[l];
+    V.nCells = 0;
+    for (int iz = 0; iz < V.nz; ++iz)
+      for (int ix = 0; ix < V.nx; ++ix)
+        for (int iy = 0; iy < V.ny; ++iy) {
+          const long long c = iy + (long long)V.NY * (ix + (long long)V.NX * iz);
+          if (!V.hEmask[c]) continue;
+          ++V.nCells;
+          for (int j = 0; j < 8; ++j)
+            V.hNmask[(iy + kLocDy[j]) + (long long)V.NY * ((ix + kLocDx[j]) + (long long)V.NX * (iz + kLocDz[j]))] = 1;
+        }
+    V.nNodes = 0;
+    for (long long n = 0; n < V.NN; ++n) V.nNodes += V.hNmask[n];
+  }
+}
+
+int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, const uint8_t* fixState, int nLoad,
+          const int* loadNodes, const double* loadVec) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    P.err = "no HIP device visible: libt3x has no CPU path";
+    return T3X_ERR_NO_DEVICE;
+  }
+  build_levels(P, vox);
+  Level& F = P.lv[0];
+  const long long NN = F.NN;
+  // ---- boundary conditions (536-557 or explicit), 1294-1336
+  std::vector<long long> fixD, loadD;
+  std::vector<uint8_t> fs;
+  std::vector<double> lv;
+  if (nFix < 0) {
+    for (int jz = 0; jz <= P.nelz; ++jz)
+      for (int jy = 0; jy <= P.nely; ++jy) fixD.push_back(jy + (long long)F.NY * (0 + (long long)F.NX * jz));
+    fs.assign(fixD.size() * 3, 1);
+    const int zTop = (int)std::llround(P.nelz / 6.0 + 1.0);
+    for (int jz = 0; jz < zTop; ++jz)
+      for (int jy = 0; jy <= P.nely; ++jy) loadD.push_back(jy + (long long)F.NY * (P.nelx + (long long)F.NX * jz));
+    const double qv = 1.0 / (double)loadD.size();
+    for (size_t i = 0; i < loadD.size(); ++i) { lv.push_back(0.0 * qv); lv.push_back(0.0 * qv); lv.push_back(-1.0 * qv); }
+  } else {
+    const long long uy = P.nely + 1, ux = P.nelx + 1;
+    auto adapt = [&](int id) {
+      const long long jy = id % uy, jx = (id / uy) % ux, jz = id / uy / ux;
+      return jy + (long long)F.NY * (jx + (long long)F.NX * jz);
+    };
+    for (int i = 0; i < nFix; ++i) {
+      fixD.push_back(adapt(fixNodes[i]));
+      for (int c = 0; c < 3; ++c) fs.push_back(fixState[3 * i + c]);
+    }
+    for (int i = 0; i < nLoad; ++i) {
+      loadD.push_back(adapt(loadNodes[i]));
+      for (int c = 0; c < 3; ++c) lv.push_back(loadVec[3 * i + c]);
+    }
+  }
+  std::vector<uint8_t> hfix(NN, 0);
+  std::vector<double> hF(3 * NN, 0.0);
+  std::vector<uint8_t> loaded(NN, 0);
+  int nL = 0, nF = 0;
+  for (size_t i = 0; i < loadD.size(); ++i) {
+    const long long d = loadD[i];
+    if (d < 0 || d >= NN || !F.hNmask[d]) continue;
+    loaded[d] = 1;
+    for (int c = 0; c < 3; ++c) hF[c * NN + d] = lv[3 * i + c];
+  }
+  for (long long d = 0; d < NN; ++d) nL += loaded[d];
+  for (size_t i = 0; i < fixD.size(); ++i) {
+    const long long d = fixD[i];
+    if (d < 0 || d >= NN || !F.hNmask[d] || loaded[d]) continue;  // loaded nodes are released (1303)
+    uint8_t m = 0;
+    for (int c = 0; c < 3; ++c) m |= (fs[3 * i + c] ? 1 : 0) << c;
+    hfix[d] = m;
+  }
+  for (long long d = 0; d < NN; ++d) nF += hfix[d] ? 1 : 0;
+  if (nL == 0) { P.err = "No Loads!"; return T3X_ERR_NO_LOADS; }
+  if (nF == 0) { P.err = "No Fixations!"; return T3X_ERR_NO_FIX; }
+
+  T3X_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  // ---- device arrays per level
+  for (int l = 0; l < P.L; ++l) {
+    Level& V = P.lv[l];
+    V.emask = P.alloc<uint8_t>(V.NN);
+    V.nmask = P.alloc<uint8_t>(V.NN);
+    T3X_HIP(hipMemcpy(V.emask, V.hEmask.data(), V.NN, hipMemcpyHostToDevice));
+    T3X_HIP(hipMemcpy(V.nmask, V.hNmask.data(), V.NN, hipMemcpyHostToDevice));
+    if (l < P.L - 1) V.diag = P.alloc<double>(3 * V.NN);
+    if (l >= 1) {
+      V.r = P.alloc<double>(3 * V.NN);
+      V.x = P.alloc<double>(3 * V.NN);
+      V.Ks = P.alloc<double>((size_t)kTri * V.NN);
+    }
+  }
+  P.fix = P.alloc<uint8_t>(NN);
+  T3X_HIP(hipMemcpy(P.fix, hfix.data(), NN, hipMemcpyHostToDevice));
+  P.F = P.alloc<double>(3 * NN);
+  T3X_HIP(hipMemcpy(P.F, hF.data(), 3 * NN * sizeof(double), hipMemcpyHostToDevice));
+  for (double** v : {&P.U, &P.r, &P.z, &P.p, &P.q, &P.tmp3}) *v = P.alloc<double>(3 * NN);
+  for (double** v : {&P.E, &P.x, &P.xPhys, &P.ce, &P.dc, &P.ca, &P.cb, &P.cc, &P.cd}) *v = P.alloc<double>(NN);
+  for (double** v : {&P.fb, &P.fx, &P.fr, &P.fz, &P.fp, &P.fq}) *v = P.alloc<double>(NN);
+  // ---- Ke and L2 templates
+  const std::vector<double> Ke = gauss_ke(P.P.nu, 1.0);
+  P.dKe = P.alloc<double>(576);
+  T3X_HIP(hipMemcpy(P.dKe, Ke.data(), 576 * sizeof(double), hipMemcpyHostToDevice));
+  const std::vector<double> T2 = l2_templates(Ke, P.lv[1].span);
+  P.dT2 = P.alloc<double>(T2.size());
+  T3X_HIP(hipMemcpy(P.dT2, T2.data(), T2.size() * sizeof(double), hipMemcpyHostToDevice));
+  // ---- L2 cells containing a fine cell that touches a fixed node (1397-1422)
+  {
+    const Level& C = P.lv[1];
+    std::vector<uint8_t> flag(C.NN, 0);
+    for (int iz = 0; iz < F.nz; ++iz)
+      for (int ix = 0; ix < F.nx; ++ix)
+        for (int iy = 0; iy < F.ny; ++iy) {
+          if (!F.hEmask[iy + (long long)F.NY * (ix + (long long)F.NX * iz)]) continue;
+          bool t = false;
+          for (int j = 0; j < 8; ++j)
+            t |= hfix[(iy + kLocDy[j]) + (long long)F.NY * ((ix + kLocDx[j]) + (long long)F.NX * (iz + kLocDz[j]))] != 0;
+          if (t) flag[iy / C.span + (long long)C.NY * (ix / C.span + (long long)C.NX * (iz / C.span))] = 1;
+        }
+    std::vector<int> list;
+    for (long long c = 0; c < C.NN; ++c)
+      if (flag[c]) list.push_back((int)c);
+    P.nBc2 = (int)list.size();
+    P.dBc2 = P.alloc<int>(list.size());
+    if (!list.empty()) T3X_HIP(hipMemcpy(P.dBc2, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+    P.dBcFlag2 = P.alloc<uint8_t>(C.NN);
+    T3X_HIP(hipMemcpy(P.dBcFlag2, flag.data(), C.NN, hipMemcpyHostToDevice));
+  }
+  // ---- coarsest band ordering: the axis permutation with the smallest bandwidth
+  {
+    const Level& C = P.lv[P.L - 1];
+    const int dims[3] = {C.NY, C.NX, C.NZ};
+    const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    std::vector<int> bestPos;
+    int bestBW = -1;
+    for (auto& pm : perms) {
+      std::vector<std::pair<long long, long long>> keys;
+      for (long long n = 0; n < C.NN; ++n) {
+        if (!C.hNmask[n]) continue;
+        const int co[3] = {(int)(n % C.NY), (int)((n / C.NY) % C.NX), (int)(n / C.NY / C.NX)};
+        keys.push_back({co[pm[0]] + (long long)dims[pm[0]] * (co[pm[1]] + (long long)dims[pm[1]] * co[pm[2]]), n});
+      }
+      std::sort(keys.begin(), keys.end());
+      std::vector<int> pos(C.NN, -1);
+      for (size_t k = 0; k < keys.size(); ++k) pos[keys[k].second] = (int)k;
+      int bw = 0;
+      for (int iz = 0; iz < C.nz; ++iz)
+        for (int ix = 0; ix < C.nx; ++ix)
+          for (int iy = 0; iy < C.ny; ++iy) {
+            if (!C.hEmask[iy + (long long)C.NY * (ix + (long long)C.NX * iz)]) continue;
+            int lo = 1 << 30, hi = -1;
+            for (int j = 0; j < 8; ++j) {
+              const int pp = pos[(iy + kLocDy[j]) + (long long)C.NY * ((ix + kLocDx[j]) + (long long)C.NX * (iz + kLocDz[j]))];
+              lo = std::min(lo, pp);
+              hi = std::max(hi, pp);
+            }
+            bw = std::max(bw, hi - lo);
+          }
+      if (bestBW < 0 || bw < bestBW) { bestBW = bw; bestPos = pos; }
+    }
+    P.cN = 3 * C.nNodes;
+    P.cBW = 3 * bestBW + 2;
+    std::vector<long long> cmap(P.cN);
+    for (long long n = 0; n < C.NN; ++n)
+      if (bestPos[n] >= 0)
+        for (int c = 0; c < 3; ++c) cmap[3 * bestPos[n] + c] = c * C.NN + n;
+    P.dCMap = P.alloc<long long>(P.cN);
+    T3X_HIP(hipMemcpy(P.dCMap, cmap.data(), P.cN * sizeof(long long), hipMemcpyHostToDevice));
+    P.dNodePos = P.alloc<int>(C.NN);
+    T3X_HIP(hipMemcpy(P.dNodePos, bestPos.data(), C.NN * sizeof(int), hipMemcpyHostToDevice));
+    P.cs.plan(P.cN, P.cBW);
+    P.band = P.alloc<double>(P.cs.band_len());
+    P.tinv = P.alloc<double>(P.cs.tinv_len());
+    P.cDinv = P.alloc<double>(P.cs.blk_len());
+    P.cDT = P.alloc<double>(P.cs.blk_len());
+    P.cG = P.alloc<double>(P.cs.blk_len());
+    P.cH = P.alloc<double>(P.cs.blk_len());
+    P.cva = P.alloc<double>(P.cs.np);
+    P.cvb = P.alloc<double>(P.cs.np);
+    P.dStatus = P.alloc<int>(1);
+  }
+  // ---- compact maps (eleMapBack / nodMapBack order)
+  for (long long n = 0; n < NN; ++n) {
+    if (F.hNmask[n]) P.hNodeMap.push_back((int)n);
+    if (F.hEmask[n]) P.hCellMap.push_back((int)n);
+  }
+  P.nNodes = (int)P.hNodeMap.size();
+  P.nCells = (int)P.hCellMap.size();
+  P.dNodeMap = P.alloc<int>(P.nNodes);
+  P.dCellMap = P.alloc<int>(P.nCells);
+  T3X_HIP(hipMemcpy(P.dNodeMap, P.hNodeMap.data(), P.nNodes * sizeof(int), hipMemcpyHostToDevice));
+  T3X_HIP(hipMemcpy(P.dCellMap, P.hCellMap.data(), P.nCells * sizeof(int), hipMemcpyHostToDevice));
+  // ---- reductions and scalars
+  P.ws.partials = P.alloc<double>((size_t)kMaxRedBlocks * 8);
+  P.ws.counter = P.alloc<unsigned int>(4);
+  P.sc = P.alloc<double>(64);
+  P.fl = P.alloc<int>(16);
+  P.oc = P.alloc<double>(8);
+  P.mm = P.alloc<double>(MM_COUNT);
+  P.dFiltKern = P.alloc<double>(128);
+  T3X_HIP(hipHostMalloc((void**)&P.hbuf, 256 * sizeof(double), hipHostMallocDefault));
+  return T3X_OK;
+}
+
+// ------------------------------------------------------------- assembly
+// Solving_AssembleFEAstencil (919-1066).
+int assemble(t3x_problem& P) {
+  hipStream_t s = P.stream;
+  const Geo g0 = geo(P.lv[0]);
+  hipEvent_t ea[2];
+  P.tbegin(TG_ASSEMBLE, ea);
+  for (int l = 1; l < P.L; ++l) {
+    Level& C = P.lv[l];
+    const Geo gc = geo(C), gf = geo(P.lv[l - 1]);
+    if (l == 1) {
+      launch_galerkin_l2(s, gf, gc, C.span, P.dT2, P.E, C.emask, P.dBcFlag2, C.Ks);
+      launch_galerkin_l2_bc(s, gf, gc, C.span, P.nBc2, P.dBc2, P.dKe, P.E, P.lv[0].emask, P.fix, C.Ks);
+    } else {
+      launch_galerkin_coarse(s, gf, gc, P.lv[l - 1].Ks, P.lv[l - 1].emask, C.emask, C.Ks);
+    }
+    if (l < P.L - 1) launch_diag_coarse(s, gc, C.Ks, C.emask, C.nmask, C.diag);
+  }
+  launch_diag_fine(s, g0, P.dKe, P.E, P.fix, P.lv[0].nmask, P.lv[0].diag);
+  const Level& C = P.lv[P.L - 1];
+  launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, P.band, (long long)P.cs.band_len());
+  P.cs.pad_identity(s, P.band);
+  T3X_HIP(hipMemsetAsync(P.dStatus, 0, sizeof(int), s));
+  P.cs.factor(s, P.band, P.tinv, P.cDinv, P.cDT, P.cG, P.cH, P.dStatus);
+  P.tend(TG_ASSEMBLE, ea);
+  int st = 0;
+  T3X_HIP(hipMemcpyAsync(&st, P.dStatus, sizeof(int), hipMemcpyDeviceToHost, s));
+  P.sync();
+  if (st) { P.err = "coarsest matrix is not positive definite"; return T3X_ERR_NOT_PD; }
+  P.assembled = true;
+  return T3X_OK;
+}
+
+// ------------------------------------------------------------- V-cycle
+// Solving_Vcycle, 'Adapted' (747-781).  red: 0 none, 1/2 fused z.r (initial / in-loop).
+void vcycle(t3x_problem& P, const double* r, double* z, double* sc, const int* fl, bool fuse) {
+  hipStream_t s = P.stream;
+  const int L = P.L;
+  hipEvent_t ev[2], ec[2];
+  P.tbegin(TG_VCYCLE, ev);
+  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, P.lv[1].nmask, P.lv[1].r, fl);
+  for (int l = 1; l < L - 1; ++l)
+    launch_restrict(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.lv[l].r, P.lv[l + 1].nmask,
+                    P.lv[l + 1].r, fl);
+  Level& C = P.lv[L - 1];
+  P.tbegin(TG_COARSE, ec);
+  P.cs.solve(s, P.dCMap, C.r, C.x, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);
+  P.tend(TG_COARSE, ec);
+  for (int l = L - 2; l >= 1; --l)
+    launch_prolong_jacobi(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.P.omega, P.lv[l + 1].x,
+                          P.lv[l].r, P.lv[l].diag, P.lv[l].nmask, P.lv[l].x, nullptr, nullptr, fl);
+  launch_prolong_jacobi(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, P.P.omega, P.lv[1].x, r, P.lv[0].diag,
+                        P.lv[0].nmask, z, fuse ? &P.ws : nullptr, sc, fl);
+  P.tend(TG_VCYCLE, ev);
+}
+
+// ------------------------------------------------------------- PCG
+// Solving_PCG (600-639) with device-side scalars and convergence flag.
+int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxIT, double* resOut, bool* warned) {
+  hipStream_t s = P.stream;
+  const long long n3 = 3 * P.NN();
+  double* sc = P.sc;
+  int* fl = P.fl;
+  const Geo g0 = geo(P.lv[0]);
+  launch_pcg_init(s, sc, fl, tol);
+  launch_norm2(s, b, n3, P.ws, &sc[SC_NORMB]);                                              // 607
+  launch_apply_K(s, g0, P.dKe, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);               // 609
+  launch_sub(s, P.r, b, P.q, n3);
+  vcycle(P, P.r, P.z, sc, nullptr, true);                                                   // 610, 612
+  launch_copy(s, P.p, P.z, n3);                                                             // 611
+  const int ce = std::max(1, P.P.checkEvery);
+  int its = maxIT;
+  bool done = false;
+  for (int it = 1; it <= maxIT; ++it) {
+    hipEvent_t ek[2];
+    P.tbegin(TG_APPLYK, ek);
+    launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                 // 614-615
+    P.tend(TG_APPLYK, ek);
+    launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                               // 616-622
+    vcycle(P, P.r, P.z, sc, fl, true);                                                      // 628-629
+    launch_pcg_pupdate(s, P.p, P.z, n3, sc, fl);                                            // 630
+    if (it % ce == 0 || it == maxIT) {
+      T3X_HIP(hipMemcpyAsync(P.hbuf, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+      P.sync();
+      const int* hf = (const int*)P.hbuf;
+      if (hf[FL_DONE]) { done = true; its = hf[FL_ITS]; break; }
+    }
+  }
+  T3X_HIP(hipMemcpyAsync(P.hbuf, sc, SC_COUNT_PCG * sizeof(double), hipMemcpyDeviceToHost, s));
+  P.sync();
+  if (resOut) *resOut = P.hbuf[SC_RES];
+  if (warned) *warned = !done;  // 634-637
+  return its;
+}
+
+// ------------------------------------------------------------- PDE filter
+void filter_setup(t3x_problem& P, int slot, double rMin) {
+  // TopOpti_SetupPDEfilter_matrixFree (816-868): iKEF = r^2 KEF0 + KEF1 with
+  // KEF0 = sum over 8 Gauss points of natural-coordinate gradient products and
+  // KEF1 the consistent mass with detJ = 1/8.
+  const double g = 1.0 / std::sqrt(3.0);
+  double K0[8][8] = {}, K1[8][8] = {};
+  for (int q = 0; q < 8; ++q) {
+    const double sq = (q & 1) ? g : -g, tq = (q & 2) ? g : -g, pq = (q & 4) ? g : -g;
+    double N[8], ds[8], dt[8], dp[8];
+    for (int a = 0; a < 8; ++a) {
+      const double sa = kLocDx[a] ? 1.0 : -1.0, ta = kLocDy[a] ? -1.0 : 1.0, pa = kLocDz[a] ? 1.0 : -1.0;
+      N[a] = 0.125 * (1 + sa * sq) * (1 + ta * tq) * (1 + pa * pq);
+      ds[a] = 0.125 * sa * (1 + ta * tq) * (1 + pa * pq);
+      dt[a] = 0.125 * ta * (1 + sa * sq) * (1 + pa * pq);
+      dp[a] = 0.125 * pa * (1 + sa * sq) * (1 + ta * tq);
+    }
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b) {
+        K0[a][b] += ds[a] * ds[b] + dt[a] * dt[b] + dp[a] * dp[b];
+        K1[a][b] += N[a] * (1.0 / 8.0) * N[b];
+      }
+  }
+  const double ir = (rMin * 1.0) / 2.0 / std::sqrt(3.0);  // 850
+  Filter& Fl = P.filt[slot];
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b) Fl.kernel[b * 8 + a] = ir * ir * K0[a][b] + K1[a][b];
+  T3X_HIP(hipMemcpy(P.dFiltKern + slot * 64, Fl.kernel, 64 * sizeof(double), hipMemcpyHostToDevice));
+  if (!Fl.diagInv) Fl.diagInv = P.alloc<double>(P.NN());
+  launch_filter_diag(P.stream, geo(P.lv[0]), P.dFiltKern + slot * 64, P.lv[0].emask, P.lv[0].nmask, Fl.diagInv);
+  Fl.ready = true;
+  Fl.r = rMin;
+}
+
+// TopOpti_ConductPDEFiltering_matrixFree (870-902); src/dst are dense cell arrays.
+int filter_run(t3x_problem& P, int slot, const double* src, double* dst) {
+  hipStream_t s = P.stream;
+  const Geo g0 = geo(P.lv[0]);
+  const long long NN = P.NN();
+  double* sc = P.sc + 16;
+  int* fl = P.fl + 4;
+  const double* kern = P.dFiltKern + slot * 64;
+  const double* dinv = P.filt[slot].diagInv;
+  launch_filter_e2n(s, g0, P.lv[0].emask, src, P.fb);  // 880-886
+  launch_pcg_init(s, sc, fl, P.P.tol / 10.0);          // 891
+  launch_norm2(s, P.fb, NN, P.ws, &sc[SC_NORMB]);
+  launch_fill(s, P.fx, 0.0, NN);
+  launch_filter_apply(s, g0, kern, P.lv[0].emask, P.fx, P.fq, nullptr, nullptr, nullptr);
+  launch_sub(s, P.fr, P.fb, P.fq, NN);
+  launch_scale_diag(s, P.fz, dinv, P.fr, NN, &P.ws, sc, nullptr);
+  launch_copy(s, P.fp, P.fz, NN);
+  const int maxIT = P.P.maxIT;
+  const int ce = std::max(1, P.P.checkEvery);
+  int its = maxIT;
+  for (int it = 1; it <= maxIT; ++it) {
+    launch_filter_apply(s, g0, kern, P.lv[0].emask, P.fp, P.fq, &P.ws, sc, fl);
+    launch_pcg_update(s, P.fx, P.fr, P.fp, P.fq, NN, P.ws, sc, fl);
+    launch_scale_diag(s, P.fz, dinv, P.fr, NN, &P.ws, sc, fl);
+    launch_pcg_pupdate(s, P.fp, P.fz, NN, sc, fl);
+    if (it % ce == 0 || it == maxIT) {
+      T3X_HIP(hipMemcpyAsync(P.hbuf, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+      P.sync();
+      const int* hf = (const int*)P.hbuf;
+      if (hf[FL_DONE]) { its = hf[FL_ITS]; break; }
+    }
+  }
+  launch_filter_n2e(s, g0, P.lv[0].emask, P.fx, dst);  // 894-901
+  return its;
+}
+
+double read_scalar(t3x_problem& P, const double* d) {
+  T3X_HIP(hipMemcpyAsync(P.hbuf, d, sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  P.sync();
+  return P.hbuf[0];
+}
+
+double masked_sum(t3x_problem& P, const double* a, const double* b, int mode) {
+  launch_masked_sum(P.stream, P.NN(), P.lv[0].emask, a, b, mode, P.ws, P.sc + 32);
+  return read_scalar(P, P.sc + 32);
+}
+
+// ------------------------------------------------------------- OC (145-155)
+double oc_update(t3x_problem& P, double* x, const double* xPhys, const double* dc, double V0, int* nBis) {
+  hipStream_t s = P.stream;
+  const double fval = masked_sum(P, xPhys, nullptr, 0) / (double)P.nCells - V0;  // 145
+  double init[8] = {0.0, 1e9, fval, 0.0, 0.0, 0.0, 0.0, 0.0};                  // 146
+  std::memcpy(P.hbuf, init, sizeof(init));
+  T3X_HIP(hipMemcpyAsync(P.oc, P.hbuf, sizeof(init), hipMemcpyHostToDevice, s));
+  for (int pass = 0; pass < 64; ++pass) {
+    launch_oc_pass(s, P.NN(), P.lv[0].emask, nullptr, x, dc, P.ws, P.oc, 3);
+    if (pass % 8 == 7) {
+      T3X_HIP(hipMemcpyAsync(P.hbuf, P.oc, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+      P.sync();
+      if (P.hbuf[3] != 0.0) break;
+    }
+  }
+  launch_oc_final(s, P.NN(), P.lv[0].emask, nullptr, x, dc, P.ws, P.oc);
+  T3X_HIP(hipMemcpyAsync(P.hbuf, P.oc, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+  P.sync();
+  if (nBis) *nBis = (int)P.hbuf[5];
+  return P.hbuf[6];
+}
+
+// ------------------------------------------------------------- MMA (1804-2049)
+void mma_alloc(t3x_problem& P) {
+  if (P.mx) return;
+  const long long NN = P.NN();
+  for (double** v : {&P.mx, &P.mxold1, &P.mxold2, &P.mlow, &P.mupp, &P.malpha, &P.mbeta, &P.mp0, &P.mq0, &P.mpij,
+                     &P.mqij, &P.mxmin, &P.mxmax})
+    *v = P.alloc<double>(NN);
+}
+
+// One MMA update on the dense arrays: P.mx holds xval on entry, the new x on exit.
+int mma_update(t3x_problem& P, int iter, const double* dfdx, double gx, const double* dgdx) {
+  hipStream_t s = P.stream;
+  const long long NN = P.NN();
+  P.hbuf[0] = gx;
+  T3X_HIP(hipMemcpyAsync(P.mm + MM_GX, P.hbuf, sizeof(double), hipMemcpyHostToDevice, s));
+  launch_mma_gensub(s, NN, P.lv[0].emask, iter, P.mx, P.mxmin, P.mxmax, P.mxold1, P.mxold2, P.mlow, P.mupp, dfdx,
+                    dgdx, P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij, P.ws, P.mm, (double)P.nCells);
+  for (int k = 0; k < 4000; ++k) {
+    launch_mma_step(s, NN, P.lv[0].emask, P.mx, P.mlow, P.mupp, P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij,
+                    P.ws, P.mm);
+    if (k % 4 == 3) {
+      T3X_HIP(hipMemcpyAsync(P.hbuf, P.mm, MM_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+      P.sync();
+      if (P.hbuf[MM_DONE] != 0.0) break;
+    }
+  }
+  T3X_HIP(hipMemcpyAsync(P.hbuf, P.mm, MM_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+  P.sync();
+  return (int)P.hbuf[MM_NEWTON];
+}
+
+// ------------------------------------------------------------- drivers
+// TOP3D_XL_TO (40-223) / TOP3D_XL_PIO (225-440) split into begin / one
+// design iteration / finish so callers (bench, serving engine) can drive
+// the loop one iteration at a time.
+int opt_begin(t3x_problem& P, int mode, double V0, int nLoop, double rMin, double rHat) {
+  hipStream_t s = P.stream;
+  const long long NN = P.NN();
+  const uint8_t* em = P.lv[0].emask;
+  OptState& S = P.opt;
+  S = OptState();
+  S.mode = mode; S.V0 = V0; S.nLoop = nLoop; S.rMin = rMin; S.rHat = rHat;
+  S.hist.clear();
+  filter_setup(P, 0, rMin);                 // 76 / 268
+  if (mode == 1) filter_setup(P, 1, rHat);  // 269
+  launch_elementwise(s, OP_SET, NN, em, P.x, nullptr, nullptr, V0, 0, 0);      // 82
+  launch_elementwise(s, OP_SET, NN, em, P.xPhys, nullptr, nullptr, V0, 0, 0);  // 83
+  launch_elementwise(s, OP_SET, NN, em, P.xTilde(), nullptr, nullptr, V0, 0, 0);  // 279
+  launch_elementwise(s, OP_SET, NN, em, P.volMax(), nullptr, nullptr, V0, 0, 0);  // 278
+  launch_fill(s, P.U, 0.0, 3 * NN);                                               // 95
+  if (mode == 1) mma_alloc(P);
+  // solid reference solve (94-106 / 294-306)
+  launch_elementwise(s, OP_SET, NN, em, P.E, nullptr, nullptr, P.P.E0, 0, 0);
+  int rc = assemble(P);
+  if (rc) return rc;
+  bool warned = false;
+  pcg_elastic(P, P.F, P.U, P.P.tol, P.P.maxIT, nullptr, &warned);
+  if (warned) P.warnings++;
+  launch_compliance(s, geo(P.lv[0]), P.dKe, em, P.U, P.ce);
+  S.cSolid = masked_sum(P, P.E, P.ce, 1);  // 104
+  S.active = true;
+  return T3X_OK;
+}
+
+// One design iteration (112-182 / 312-400).  Returns 1 when the loop
+// condition (111 / 311) stops the optimisation, 0 otherwise.
+int opt_step(t3x_problem& P, t3x_iter* it) {
+  OptState& S = P.opt;
+  if (!S.active) { P.err = "t3x_opt_step without t3x_opt_begin"; return -T3X_ERR_INPUT; }
+  if (!(S.loop < S.nLoop && S.change > 0.0001 && S.sharp > 0.01)) return 1;  // 111 / 311
+  hipStream_t s = P.stream;
+  const long long NN = P.NN();
+  const uint8_t* em = P.lv[0].emask;
+  const double E0 = P.P.E0, Emin = P.P.Emin, V0 = S.V0;
+  const double nE = (double)P.nCells;
+  const double cSolid = S.cSolid;
+  const double eta = 0.5;
+  double* tmp = P.cc;
+  const double t0 = wall();
+  ++S.loop;
+  ++S.loopbeta;
+  launch_simp(s, NN, em, P.xPhys, Emin, E0, P.E);  // 116
+  P.sync();
+  const double ta = wall();
+  int rc = assemble(P);  // 118
+  if (rc) return -rc;
+  const double tb = wall();
+  bool warned = false;
+  double resFinal = 0.0;
+  const int its = pcg_elastic(P, P.F, P.U, P.P.tol, P.P.maxIT, &resFinal, &warned);  // 121
+  if (warned) P.warnings++;
+  const double tcg = wall();
+  launch_compliance(s, geo(P.lv[0]), P.dKe, em, P.U, P.ce);                  // 124
+  launch_elementwise(s, OP_DIVC, NN, em, tmp, P.ce, nullptr, cSolid, 0, 0);  // 125 ceNorm
+  const double cObj = masked_sum(P, P.E, tmp, 1);                            // 126
+  const double cDesign = cObj * cSolid;                                      // 127
+  const double V = masked_sum(P, P.xPhys, nullptr, 0) / nE;                  // 128
+  launch_elementwise(s, OP_DC, NN, em, P.dc, P.xPhys, P.ce, P.P.penal * (E0 - Emin), cSolid, 0);  // 129
+  double tOpt = wall() - tcg, tFilt = 0.0, tLVF = 0.0, fval = 0.0;
+  int fIts = 0;
+  if (S.mode == 0) {
+    const double t1 = wall();
+    launch_elementwise(s, OP_MUL, NN, em, tmp, P.x, P.dc, 0, 0, 0);  // 137
+    fIts += filter_run(P, 0, tmp, P.dc);
+    launch_elementwise(s, OP_DIVMAX, NN, em, P.dc, P.x, nullptr, 0, 0, 0);
+    P.sync();
+    const double t2 = wall();
+    tFilt += t2 - t1;
+    fval = masked_sum(P, P.xPhys, nullptr, 0) / nE - V0;  // 145
+    S.change = oc_update(P, P.x, P.xPhys, P.dc, V0, nullptr);  // 146-155
+    launch_copy(s, P.xPhys, P.x, NN);  // 160
+    P.sync();
+    tOpt += wall() - t2;
+  } else {
+    const int pp = 16;  // pPIO (235)
+    double* xhat = P.cd;
+    double* xTilde = P.xTilde();
+    double* volMax = P.volMax();
+    const double t1 = wall();
+    fIts += filter_run(P, 1, P.xPhys, xhat);  // 334
+    const double sp = masked_sum(P, xhat, volMax, 3) / nE;
+    const double fac = std::pow(sp, 1.0 / pp - 1.0);
+    double* dfdx_pde = P.mxmin;  // scratch until the MMA bounds are set
+    launch_elementwise(s, OP_PNORMGRAD, NN, em, dfdx_pde, xhat, volMax, fac, nE, 0);  // 335
+    P.sync();
+    const double t2 = wall();
+    tLVF += t2 - t1;
+    double* dx = P.mxmax;
+    launch_elementwise(s, OP_DHEAVI, NN, em, dx, xTilde, nullptr, S.beta, eta, 0);  // 340
+    launch_elementwise(s, OP_MUL, NN, em, tmp, P.dc, dx, 0, 0, 0);
+    fIts += filter_run(P, 0, tmp, P.dc);  // 341
+    P.sync();
+    const double t3 = wall();
+    tFilt += t3 - t2;
+    double* dfdx = P.tmp3;
+    fIts += filter_run(P, 1, dfdx_pde, dfdx);  // 346
+    P.sync();
+    const double t4 = wall();
+    tLVF += t4 - t3;
+    launch_elementwise(s, OP_MUL, NN, em, tmp, dfdx, dx, 0, 0, 0);
+    fIts += filter_run(P, 0, tmp, dfdx);  // 350
+    P.sync();
+    const double t5 = wall();
+    tFilt += t5 - t4;
+    // MMA (354-369); every cell is active (no passive cells in these models)
+    if (S.loop == 1) {  // 355
+      launch_copy(s, P.mxold1, P.x, NN);
+      launch_copy(s, P.mxold2, P.x, NN);
+      launch_fill(s, P.mlow, 0.0, NN);
+      launch_fill(s, P.mupp, 0.0, NN);
+    }
+    fval = std::pow(sp, 1.0 / pp) - 1.0;                                                 // 360
+    launch_elementwise(s, OP_MMA_BOUNDS, NN, em, P.mxmin, P.x, nullptr, 0.1, 0.0, 0);  // 364
+    launch_elementwise(s, OP_MMA_BOUNDS, NN, em, P.mxmax, P.x, nullptr, 0.1, 1.0, 0);  // 365
+    launch_copy(s, P.mx, P.x, NN);
+    mma_update(P, S.loop, P.dc, fval, dfdx);  // 367
+    launch_copy(s, tmp, P.x, NN);             // xval
+    launch_copy(s, P.x, P.mx, NN);            // 369
+    launch_masked_maxdiff(s, NN, em, P.x, tmp, P.ws, P.sc + 33);  // 368
+    S.change = read_scalar(P, P.sc + 33);
+    const double t6 = wall();
+    tOpt += t6 - t5;
+    fIts += filter_run(P, 0, P.x, xTilde);                                          // 374
+    if (S.loopbeta >= 40) { S.beta = std::min(2 * S.beta, 128.0); S.loopbeta = 0; }  // 375-379
+    launch_elementwise(s, OP_HEAVI, NN, em, P.xPhys, xTilde, nullptr, S.beta, eta, 0);  // 380
+    P.sync();
+    tFilt += wall() - t6;
+  }
+  S.sharp = 4.0 * masked_sum(P, P.xPhys, nullptr, 2) / nE;  // 166 / 382
+  const double tTot = wall() - t0;
+  t3x_iter r{};
+  r.loop = S.loop; r.c = cDesign; r.vol = V; r.cons = fval; r.sharp = S.sharp; r.change = S.change;
+  r.pcgIts = its; r.filterIts = fIts; r.res = resFinal;
+  r.t[0] = tb - ta; r.t[1] = tcg - tb; r.t[2] = tOpt; r.t[3] = tFilt; r.t[4] = tLVF; r.t[5] = tTot;
+  S.hist.push_back(r);
+  if (it) *it = r;
+  return (S.loop < S.nLoop && S.change > 0.0001 && S.sharp > 0.01) ? 0 : 1;
+}
+
+int opt_result(t3x_problem& P, t3x_result* O) {
+  hipStream_t s = P.stream;
+  const long long NN = P.NN();
+  const OptState& S = P.opt;
+  const int n = (int)S.hist.size();
+  for (int k = 0; k < n; ++k) {
+    const t3x_iter& r = S.hist[k];
+    if (O->cHist) O->cHist[k] = r.c;
+    if (O->volHist) O->volHist[k] = r.vol;
+    if (O->consHist) O->consHist[k] = r.cons;
+    if (O->sharpHist) O->sharpHist[k] = r.sharp;
+    if (O->lssIts) O->lssIts[k] = r.pcgIts;
+    if (O->resHist) O->resHist[k] = r.res;
+    if (O->tHist) std::memcpy(O->tHist + (size_t)k * 6, r.t, 6 * sizeof(double));
+  }
+  if (O->xPhys) {
+    launch_gather_compact(s, P.dCellMap, P.nCells, 1, NN, P.xPhys, P.tmp3);
+    T3X_HIP(hipMemcpyAsync(O->xPhys, P.tmp3, P.nCells * sizeof(double), hipMemcpyDeviceToHost, s));
+    P.sync();
+  }
+  O->cSolid = S.cSolid;
+  O->loops = n;
+  O->warnings = P.warnings;
+  return T3X_OK;
+}
+
+int run(t3x_problem& P, int mode, double V0, int nLoop, double rMin, double rHat, t3x_result* O) {
+  int rc = opt_begin(P, mode, V0, nLoop, rMin, rHat);
+  if (rc) return rc;
+  for (;;) {
+    const int k = P.opt.loop;
+    rc = opt_step(P, nullptr);
+    if (rc < 0) return -rc;
+    if (O->xSnap && k < O->nSnap && P.opt.loop > k) {
+      launch_gather_compact(P.stream, P.dCellMap, P.nCells, 1, P.NN(), P.xPhys, P.tmp3);
+      T3X_HIP(hipMemcpyAsync(O->xSnap + (size_t)k * P.nCells, P.tmp3, P.nCells * sizeof(double),
+                             hipMemcpyDeviceToHost, P.stream));
+      P.sync();
+    }
+    if (rc == 1) break;
+  }
+  return opt_result(P, O);
+}
+
+// Upload a compact nodal vector (3 per node) into a zeroed dense SoA array.
+void upload_nodal(t3x_problem& P, const double* h, double* d) {
+  hipStream_t s = P.stream;
+  launch_fill(s, d, 0.0, 3 * P.NN());
+  double* stage = P.tmp3;  // compact staging (3*nNodes <= 3*NN)
+  T3X_HIP(hipMemcpyAsync(stage, h, 3 * (size_t)P.nNodes * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_scatter_compact(s, P.dNodeMap, P.nNodes, 3, P.NN(), stage, d);
+}
+void download_nodal(t3x_problem& P, const double* d, double* h) {
+  hipStream_t s = P.stream;
+  launch_gather_compact(s, P.dNodeMap, P.nNodes, 3, P.NN(), d, P.tmp3);
+  T3X_HIP(hipMemcpyAsync(h, P.tmp3, 3 * (size_t)P.nNodes * sizeof(double), hipMemcpyDeviceToHost, s));
+  P.sync();
+}
+void upload_cells(t3x_problem& P, const double* h, double* d) {
+  hipStream_t s = P.stream;
+  launch_fill(s, d, 0.0, P.NN());
+  T3X_HIP(hipMemcpyAsync(P.tmp3, h, (size_t)P.nCells * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_scatter_compact(s, P.dCellMap, P.nCells, 1, P.NN(), P.tmp3, d);
+}
+void download_cells(t3x_problem& P, const double* d, double* h) {
+  hipStream_t s = P.stream;
+  launch_gather_compact(s, P.dCellMap, P.nCells, 1, P.NN(), d, P.tmp3);
+  T3X_HIP(hipMemcpyAsync(h, P.tmp3, (size_t)P.nCells * sizeof(double), hipMemcpyDeviceToHost, s));
+  P.sync();
+}
+
+template <class F>
+int guarded(t3x_problem* h, F f) {
+  if (!h) return T3X_ERR_INPUT;
+  try {
+    return f();
+  } catch (const HipError& e) {
+    h->err = e.msg;
+    return T3X_ERR_HIP;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return T3X_ERR_HIP;
+  }
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+void t3x_default_params(t3x_params* p) {
+  p->E0 = 1.0;
+  p->nu = 0.3;
+  p->Emin = 1.0e-6;
+  p->penal = 3.0;
+  p->tol = 1.0e-3;
+  p->maxIT = 800;
+  p->omega = 0.6;
+  p->coarsestControl = 50000;
+  p->precision = 0;
+  p->checkEvery = 4;
+  p->timing = 0;
+}
+
+int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+               const uint8_t* fixState, int nLoad, const int* loadNodes, const double* loadVec,
+               const t3x_params* params, t3x_problem** out) {
+  if (!out) return T3X_ERR_INPUT;
+  *out = nullptr;
+  std::unique_ptr<t3x_problem> P(new t3x_problem());
+  if (params) P->P = *params; else t3x_default_params(&P->P);
+  P->nely = nely; P->nelx = nelx; P->nelz = nelz;
+  if (!voxels || nelx < 3 || nely < 3 || nelz < 3) {  // 532-534
+    P->err = "Inappropriate Input Model!";
+    *out = P.release();
+    return T3X_ERR_INPUT;
+  }
+  int rc;
+  try {
+    rc = setup(*P, voxels, nFix, fixNodes, fixState, nLoad, loadNodes, loadVec);
+  } catch (const HipError& e) {
+    P->err = e.msg;
+    rc = T3X_ERR_HIP;
+  }
+  *out = P.release();
+  return rc;
+}
+
+void t3x_destroy(t3x_problem* h) { delete h; }
+
+const char* t3x_last_error(const t3x_problem* h) { return h ? h->err.c_str() : "null handle"; }
+
+int t3x_info(const t3x_problem* h, int* info) {
+  if (!h || !info) return T3X_ERR_INPUT;
+  info[0] = h->L;
+  for (int l = 0; l < h->L; ++l) {
+    const Level& V = h->lv[l];
+    int* o = info + 1 + 8 * l;
+    o[0] = V.nx; o[1] = V.ny; o[2] = V.nz; o[3] = V.span; o[4] = V.nCells; o[5] = V.nNodes; o[6] = 0; o[7] = 0;
+  }
+  return T3X_OK;
+}
+
+int t3x_run(t3x_problem* h, int mode, double V0, int nLoop, double rMin, double rHat, t3x_result* out) {
+  return guarded(h, [&] {
+    if (mode != 0 && mode != 1) { h->err = "Wrong Input!"; return (int)T3X_ERR_INPUT; }
+    t3x_result dummy{};
+    return run(*h, mode, V0, nLoop, rMin, rHat, out ? out : &dummy);
+  });
+}
+
+int t3x_opt_begin(t3x_problem* h, int mode, double V0, int nLoop, double rMin, double rHat) {
+  return guarded(h, [&] {
+    if (mode != 0 && mode != 1) { h->err = "Wrong Input!"; return (int)T3X_ERR_INPUT; }
+    return opt_begin(*h, mode, V0, nLoop, rMin, rHat);
+  });
+}
+
+int t3x_opt_step(t3x_problem* h, t3x_iter* it, int* finished) {
+  return guarded(h, [&] {
+    const int rc = opt_step(*h, it);
+    if (rc < 0) return -rc;
+    if (finished) *finished = rc;
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_opt_result(t3x_problem* h, t3x_result* out) {
+  return guarded(h, [&] { return opt_result(*h, out); });
+}
+
+int t3x_kernel_time(t3x_problem* h, int group, long long* count, double* ms) {
+  return guarded(h, [&] {
+    if (group < 0 || group >= TG_COUNT) return (int)T3X_ERR_INPUT;
+    h->tflush(group);
+    if (count) *count = h->timers[group].count;
+    if (ms) *ms = h->timers[group].ms;
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_kernel_time_reset(t3x_problem* h) {
+  return guarded(h, [&] {
+    for (int g = 0; g < TG_COUNT; ++g) {
+      h->tflush(g);
+      h->timers[g].ms = 0.0;
+      h->timers[g].count = 0;
+    }
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_set_modulus(t3x_problem* h, const double* E) {
+  return guarded(h, [&] {
+    upload_cells(*h, E, h->E);
+    h->sync();
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_assemble(t3x_problem* h) { return guarded(h, [&] { return assemble(*h); }); }
+
+int t3x_apply_K(t3x_problem* h, const double* u, double* y) {
+  return guarded(h, [&] {
+    upload_nodal(*h, u, h->p);
+    launch_apply_K(h->stream, geo(h->lv[0]), h->dKe, h->E, h->fix, h->p, h->q, h->ws, h->sc, nullptr, false);
+    download_nodal(*h, h->q, y);
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_vcycle(t3x_problem* h, const double* r, double* x) {
+  return guarded(h, [&] {
+    if (!h->assembled) { h->err = "t3x_vcycle before t3x_assemble"; return (int)T3X_ERR_INPUT; }
+    upload_nodal(*h, r, h->r);
+    vcycle(*h, h->r, h->z, h->sc, nullptr, false);
+    download_nodal(*h, h->z, x);
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_pcg(t3x_problem* h, const double* b, double* x, double tol, int maxIT, int* its, double* resnorm) {
+  return guarded(h, [&] {
+    if (!h->assembled) { h->err = "t3x_pcg before t3x_assemble"; return (int)T3X_ERR_INPUT; }
+    if (!h->bvec) h->bvec = h->alloc<double>(3 * h->NN());
+    upload_nodal(*h, b, h->bvec);
+    upload_nodal(*h, x, h->U);
+    bool warned = false;
+    double res = 0.0;
+    const int it = pcg_elastic(*h, h->bvec, h->U, tol, maxIT, &res, &warned);
+    download_nodal(*h, h->U, x);
+    if (its) *its = it;
+    if (resnorm) *resnorm = res;
+    return warned ? (int)T3X_WARN_MAXIT : (int)T3X_OK;
+  });
+}
+
+int t3x_compliance(t3x_problem* h, const double* U, double* ce) {
+  return guarded(h, [&] {
+    upload_nodal(*h, U, h->p);
+    launch_compliance(h->stream, geo(h->lv[0]), h->dKe, h->lv[0].emask, h->p, h->ce);
+    download_cells(*h, h->ce, ce);
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_filter_setup(t3x_problem* h, int slot, double radius) {
+  return guarded(h, [&] {
+    if (slot < 0 || slot > 1) return (int)T3X_ERR_INPUT;
+    filter_setup(*h, slot, radius);
+    h->sync();
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_filter(t3x_problem* h, int slot, const double* src, double* dst, int* its) {
+  return guarded(h, [&] {
+    if (slot < 0 || slot > 1 || !h->filt[slot].ready) return (int)T3X_ERR_INPUT;
+    upload_cells(*h, src, h->ca);
+    const int it = filter_run(*h, slot, h->ca, h->cb);
+    download_cells(*h, h->cb, dst);
+    if (its) *its = it;
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_oc(t3x_problem* h, double* x, const double* xPhys, const double* dc, double V0, double* change,
+           int* nBisect) {
+  return guarded(h, [&] {
+    upload_cells(*h, x, h->ca);
+    upload_cells(*h, xPhys, h->cb);
+    upload_cells(*h, dc, h->cc);
+    const double ch = oc_update(*h, h->ca, h->cb, h->cc, V0, nBisect);
+    download_cells(*h, h->ca, x);
+    if (change) *change = ch;
+    return (int)T3X_OK;
+  });
+}
+
+int t3x_mma(t3x_problem* h, int iter, double* xval, const double* xmin, const double* xmax, double* xold1,
+            double* xold2, const double* dfdx, double gx, const double* dgdx, double* low, double* upp,
+            int* nNewton) {
+  return guarded(h, [&] {
+    mma_alloc(*h);
+    upload_cells(*h, xval, h->mx);
+    upload_cells(*h, xmin, h->mxmin);
+    upload_cells(*h, xmax, h->mxmax);
+    upload_cells(*h, xold1, h->mxold1);
+    upload_cells(*h, xold2, h->mxold2);
+    upload_cells(*h, low, h->mlow);
+    upload_cells(*h, upp, h->mupp);
+    upload_cells(*h, dfdx, h->ca);
+    upload_cells(*h, dgdx, h->cb);
+    const int nn = mma_update(*h, iter, h->ca, gx, h->cb);
+    download_cells(*h, h->mx, xval);
+    download_cells(*h, h->mxold1, xold1);
+    download_cells(*h, h->mxold2, xold2);
+    download_cells(*h, h->mlow, low);
+    download_cells(*h, h->mupp, upp);
+    if (nNewton) *nNewton = nn;
+    return (int)T3X_OK;
+  });
+}
+
+}  // extern "C"

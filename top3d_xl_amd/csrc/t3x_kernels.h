@@ -1,0 +1,114 @@
+// Kernel launchers of libt3x (all asynchronous on the given stream).
+#pragma once
+
+#include "t3x_internal.h"
+
+namespace t3x {
+
+// Scalar slots in device memory (double sc[] / int fl[]) for the solvers.
+enum ScalarSlot {
+  SC_NORMB = 0, SC_ZR, SC_PQ, SC_RR, SC_ZRN, SC_RES, SC_LAMBDA, SC_BETA,
+  SC_TOL, SC_COUNT_PCG  // one PCG block = SC_COUNT_PCG doubles
+};
+enum FlagSlot { FL_DONE = 0, FL_ITS, FL_ITER, FL_COUNT };
+
+// Geometry passed by value to kernels.
+struct Geo {
+  int ny, nx, nz;  // cells
+  int NY, NX, NZ;  // nodes
+  long long NN;
+};
+inline Geo geo(const Level& L) { return Geo{L.ny, L.nx, L.nz, L.NY, L.NX, L.NZ, L.NN}; }
+
+// Reduction workspace: partials (kMaxRedBlocks * 8 doubles) + one counter.
+struct RedWs {
+  double* partials;
+  unsigned int* counter;
+};
+
+// ---------------------------------------------------------------- FEM
+void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                    const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
+void launch_compliance(hipStream_t s, const Geo& g, const double* Ke, const uint8_t* emask, const double* u,
+                       double* ce);
+void launch_diag_fine(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                      const uint8_t* nmask, double* diag);
+void launch_diag_coarse(hipStream_t s, const Geo& g, const double* Ks, const uint8_t* emask,
+                        const uint8_t* nmask, double* diag);
+void launch_galerkin_l2(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* T, const double* E,
+                        const uint8_t* emaskC, const uint8_t* bcFlagC, double* Ks);
+void launch_galerkin_l2_bc(hipStream_t s, const Geo& gf, const Geo& gc, int span, int nBc, const int* bcCells,
+                           const double* Ke, const double* E, const uint8_t* emaskF, const uint8_t* fix,
+                           double* Ks);
+void launch_galerkin_coarse(hipStream_t s, const Geo& gf, const Geo& gc, const double* KsF, const uint8_t* emaskF,
+                            const uint8_t* emaskC, double* KsC);
+
+// ---------------------------------------------------------------- MG
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, const uint8_t* nmaskC,
+                     double* rc, const int* fl);
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
+                           const double* rf, const double* diag, const uint8_t* nmaskF, double* out,
+                           const RedWs* ws, double* sc, const int* fl);
+void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl);
+void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long NN3,
+                           const int* fl);
+void launch_band_assemble(hipStream_t s, const Geo& g, const double* Ks, const uint8_t* emask, const int* nodePos,
+                          int bw, double* band, long long nBand);
+int launch_band_factor(hipStream_t s, int n, int bw, double* band, int* status);
+void launch_band_post(hipStream_t s, int n, int bw, int nb, const double* band, double* bandT, double* dinv);
+void launch_band_solve(hipStream_t s, int n, int bw, int nb, const double* band, const double* bandT,
+                       const double* dinv, double* v, const int* fl);
+
+// ---------------------------------------------------------------- vectors
+void launch_fill(hipStream_t s, double* p, double v, long long n);
+void launch_copy(hipStream_t s, double* dst, const double* src, long long n);
+void launch_sub(hipStream_t s, double* r, const double* b, const double* q, long long n);  // r = b - q
+void launch_norm2(hipStream_t s, const double* v, long long n, const RedWs& ws, double* out);
+void launch_dot(hipStream_t s, const double* a, const double* b, long long n, const RedWs& ws, double* out);
+void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, const double* q, long long n,
+                       const RedWs& ws, double* sc, int* fl);
+void launch_pcg_pupdate(hipStream_t s, double* p, const double* z, long long n, const double* sc, const int* fl);
+void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol);
+void launch_scale_diag(hipStream_t s, double* z, const double* dinv, const double* r, long long n,
+                       const RedWs* ws, double* sc, const int* fl);
+void launch_gather_compact(hipStream_t s, const int* map, int n, int ncomp, long long NN, const double* dense,
+                           double* compact);
+void launch_scatter_compact(hipStream_t s, const int* map, int n, int ncomp, long long NN, const double* compact,
+                            double* dense);
+
+// ---------------------------------------------------------------- filter
+void launch_filter_e2n(hipStream_t s, const Geo& g, const uint8_t* emask, const double* src, double* b);
+void launch_filter_apply(hipStream_t s, const Geo& g, const double* kern, const uint8_t* emask, const double* u,
+                         double* y, const RedWs* ws, double* sc, const int* fl);
+void launch_filter_n2e(hipStream_t s, const Geo& g, const uint8_t* emask, const double* u, double* tar);
+void launch_filter_diag(hipStream_t s, const Geo& g, const double* kern, const uint8_t* emask,
+                        const uint8_t* nmask, double* dinv);
+
+// ---------------------------------------------------------------- optimiser
+void launch_simp(hipStream_t s, long long n, const uint8_t* emask, const double* xPhys, double Emin, double E0,
+                 double* E);
+void launch_masked_sum(hipStream_t s, long long n, const uint8_t* emask, const double* a, const double* b,
+                       int mode, const RedWs& ws, double* out);
+void launch_masked_maxdiff(hipStream_t s, long long n, const uint8_t* emask, const double* a, const double* b,
+                           const RedWs& ws, double* out);
+void launch_oc_pass(hipStream_t s, long long n, const uint8_t* emask, const uint8_t* passive, const double* x,
+                    const double* dc, const RedWs& ws, double* oc, int depth);
+void launch_oc_final(hipStream_t s, long long n, const uint8_t* emask, const uint8_t* passive, double* x,
+                     const double* dc, const RedWs& ws, double* oc);
+void launch_elementwise(hipStream_t s, int op, long long n, const uint8_t* emask, double* a, const double* b,
+                        const double* c, double p0, double p1, double p2);
+enum { OP_DC = 0, OP_MUL, OP_DIVMAX, OP_HEAVI, OP_DHEAVI, OP_PNORMGRAD, OP_COPY, OP_SET, OP_DIVC,
+       OP_MMA_BOUNDS };
+enum MmaSlot {
+  MM_LAM = 0, MM_MU, MM_EPSI, MM_ERR, MM_LOOP, MM_DONE, MM_B, MM_Y, MM_Z, MM_GRAD, MM_HESS, MM_S1, MM_S2,
+  MM_TOL, MM_C, MM_A, MM_NEWTON, MM_GX, MM_COUNT
+};
+void launch_mma_gensub(hipStream_t s, long long n, const uint8_t* emask, int iter, const double* x,
+                       const double* xmin, const double* xmax, double* xold1, double* xold2, double* low,
+                       double* upp, const double* dfdx, const double* dgdx, double* alpha, double* beta,
+                       double* p0, double* q0, double* pij, double* qij, const RedWs& ws, double* mm, double nAct);
+void launch_mma_step(hipStream_t s, long long n, const uint8_t* emask, double* x, const double* low,
+                     const double* upp, const double* alpha, const double* beta, const double* p0, const double* q0,
+                     const double* pij, const double* qij, const RedWs& ws, double* mm);
+
+}  // namespace t3x

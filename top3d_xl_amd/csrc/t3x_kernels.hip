@@ -1,0 +1,1435 @@
+// HIP kernels of libt3x for gfx950 (CDNA4, wave64).
+//
+// FP64 "plain" path.  Reference lines are TOP3D_XL.m.
+#include <hip/hip_runtime.h>
+
+#include "t3x_kernels.h"
+#include "t3x_reduce.h"
+
+namespace t3x {
+
+namespace {
+
+__device__ __forceinline__ void node_yxz(long long n, const Geo& g, int& y, int& x, int& z) {
+  const unsigned int nn = (unsigned int)n;
+  const unsigned int t = nn / (unsigned int)g.NY;
+  y = (int)(nn - t * (unsigned int)g.NY);
+  z = (int)(t / (unsigned int)g.NX);
+  x = (int)(t - (unsigned int)z * (unsigned int)g.NX);
+}
+
+__device__ __forceinline__ long long nidx(const Geo& g, int y, int x, int z) {
+  return (long long)y + (long long)g.NY * ((long long)x + (long long)g.NX * z);
+}
+
+__device__ __forceinline__ bool cell_in(const Geo& g, int cy, int cx, int cz) {
+  return cy >= 0 && cx >= 0 && cz >= 0 && cy < g.ny && cx < g.nx && cz < g.nz;
+}
+
+inline int grid_for(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > (1LL << 30)) b = 1LL << 30;
+  return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+// =========================================================================
+// One-block finalisers: fold a reduction's partials (fixed order) and apply
+// the scalar logic of the solver step that consumes it.
+// =========================================================================
+enum FinOp { FIN_PQ = 0, FIN_ZR_INIT, FIN_ZR_LOOP, FIN_RR, FIN_SUM, FIN_SQRT };
+
+__global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* __restrict__ partials, double* sc,
+                                                 int* fl, double* out) {
+  if (fl && fl[FL_DONE]) return;
+  double v[1];
+  fold_partials<1>(partials, nb, v);
+  if (threadIdx.x != 0) return;
+  const double t = v[0];
+  switch (op) {
+    case FIN_PQ:  // 615
+      sc[SC_PQ] = t;
+      sc[SC_LAMBDA] = sc[SC_ZR] / t;
+      break;
+    case FIN_ZR_INIT:  // 612
+      sc[SC_ZR] = t;
+      break;
+    case FIN_ZR_LOOP:  // 629-631
+      sc[SC_ZRN] = t;
+      sc[SC_BETA] = t / sc[SC_ZR];
+      sc[SC_ZR] = t;
+      break;
+    case FIN_RR: {  // 618-622
+      const double res = sqrt(t) / sc[SC_NORMB];
+      sc[SC_RR] = t;
+      sc[SC_RES] = res;
+      const int it = fl[FL_ITER] + 1;
+      fl[FL_ITER] = it;
+      if (res < sc[SC_TOL]) {
+        fl[FL_DONE] = 1;
+        fl[FL_ITS] = it;
+      }
+    } break;
+    case FIN_SUM: *out = t; break;
+    case FIN_SQRT: *out = sqrt(t); break;
+    default: break;
+  }
+}
+static void fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out) {
+  hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBlock), 0, s, op, nb, ws.partials, sc, fl, out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fin_max(int nb, const double* __restrict__ partials, double* out) {
+  const double m = fold_max(partials, nb);
+  if (threadIdx.x == 0) *out = m;
+}
+
+// =========================================================================
+// K*u on the finest level (Solving_KbyU_MatrixFree, 641-726).
+//
+// Node-centric gather: y_n = sum over the <= 8 cells around n of
+// E_e * Ke[rows of n] * u_e, with the fixed DOFs handled as
+// P_free K(E) P_free + I_fixed — identical to the per-element
+// uniqueKesFree_*E + uniqueKesFixed_ construction (1338-1437, 703-715):
+// the fixed rows/cols of every touching element are zero and the 1/nAdj
+// diagonal entries of the touching elements add up to 1.
+// =========================================================================
+__global__ __launch_bounds__(kBlock) void k_apply_K(Geo g, const double* __restrict__ Ke,
+                                                    const double* __restrict__ E,
+                                                    const uint8_t* __restrict__ fix,
+                                                    const double* __restrict__ u, double* __restrict__ y,
+                                                    RedWs ws, double* sc, const int* fl, int pcgStep) {
+  if (fl && fl[FL_DONE]) return;
+  const long long NN = g.NN;
+  double pq = 0.0;
+  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < NN; n += (long long)gridDim.x * kBlock) {
+    int iy, ix, iz;
+    node_yxz(n, g, iy, ix, iz);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;  // offset of n inside the cell
+      const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
+      if (!cell_in(g, cy, cx, cz)) continue;
+      const long long ce = nidx(g, cy, cx, cz);
+      const double Ee = E[ce];
+      if (Ee == 0.0) continue;
+      const int jn = loc_index(dy, dx, dz);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const long long nb = nidx(g, cy + kLocDy[j], cx + kLocDx[j], cz + kLocDz[j]);
+        const uint8_t f = fix[nb];
+        const double v0 = (f & 1) ? 0.0 : u[nb];
+        const double v1 = (f & 2) ? 0.0 : u[NN + nb];
+        const double v2 = (f & 4) ? 0.0 : u[2 * NN + nb];
+        s0 += Ke[(3 * jn + 0) * 24 + 3 * j + 0] * v0 + Ke[(3 * jn + 0) * 24 + 3 * j + 1] * v1 +
+              Ke[(3 * jn + 0) * 24 + 3 * j + 2] * v2;
+        s1 += Ke[(3 * jn + 1) * 24 + 3 * j + 0] * v0 + Ke[(3 * jn + 1) * 24 + 3 * j + 1] * v1 +
+              Ke[(3 * jn + 1) * 24 + 3 * j + 2] * v2;
+        s2 += Ke[(3 * jn + 2) * 24 + 3 * j + 0] * v0 + Ke[(3 * jn + 2) * 24 + 3 * j + 1] * v1 +
+              Ke[(3 * jn + 2) * 24 + 3 * j + 2] * v2;
+      }
+      a0 += s0 * Ee;
+      a1 += s1 * Ee;
+      a2 += s2 * Ee;
+    }
+    const uint8_t f = fix[n];
+    const double u0 = u[n], u1 = u[NN + n], u2 = u[2 * NN + n];
+    const double y0 = (f & 1) ? u0 : a0, y1 = (f & 2) ? u1 : a1, y2 = (f & 4) ? u2 : a2;
+    y[n] = y0;
+    y[NN + n] = y1;
+    y[2 * NN + n] = y2;
+    pq += u0 * y0 + u1 * y1 + u2 * y2;
+  }
+  if (pcgStep) {
+    double v[1] = {pq};
+    write_partials<1>(v, ws.partials);
+  }
+}
+
+void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                    const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
+  const int nb = pcgStep ? red_blocks(g.NN) : grid_for(g.NN);
+  hipLaunchKernelGGL(k_apply_K, dim3(nb), dim3(kBlock), 0, s, g, Ke, E, fix, u, y, ws, sc, fl, pcgStep ? 1 : 0);
+  if (pcgStep) fin1(s, FIN_PQ, nb, ws, sc, (int*)fl, nullptr);
+}
+
+// =========================================================================
+// Unit compliance ce_e = u_e^T Ke u_e with the unmodified Ke (579-598).
+// =========================================================================
+__global__ __launch_bounds__(kBlock) void k_compliance(Geo g, const double* __restrict__ Ke,
+                                                       const uint8_t* __restrict__ emask,
+                                                       const double* __restrict__ u, double* __restrict__ ce) {
+  const long long NN = g.NN;
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= NN) return;
+  if (!emask[n]) { ce[n] = 0.0; return; }
+  int iy, ix, iz;
+  node_yxz(n, g, iy, ix, iz);
+  double ue[24];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const long long nb = nidx(g, iy + kLocDy[j], ix + kLocDx[j], iz + kLocDz[j]);
+    ue[3 * j] = u[nb];
+    ue[3 * j + 1] = u[NN + nb];
+    ue[3 * j + 2] = u[2 * NN + nb];
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 24; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int b = 0; b < 24; ++b) t += ue[b] * Ke[k * 24 + b];
+    s += t * ue[k];
+  }
+  ce[n] = s;
+}
+
+void launch_compliance(hipStream_t s, const Geo& g, const double* Ke, const uint8_t* emask, const double* u,
+                       double* ce) {
+  hipLaunchKernelGGL(k_compliance, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, Ke, emask, u, ce);
+}
+
+// =========================================================================
+// Jacobi diagonals (1001-1050).
+// =========================================================================
+__global__ void k_diag_fine(Geo g, const double* __restrict__ Ke, const double* __restrict__ E,
+                            const uint8_t* __restrict__ fix, const uint8_t* __restrict__ nmask,
+                            double* __restrict__ diag) {
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.NN) return;
+  double d[3] = {0.0, 0.0, 0.0};
+  if (nmask[n]) {
+    int iy, ix, iz;
+    node_yxz(n, g, iy, ix, iz);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
+      const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
+      if (!cell_in(g, cy, cx, cz)) continue;
+      const double Ee = E[nidx(g, cy, cx, cz)];
+      if (Ee == 0.0) continue;
+      const int jn = loc_index(dy, dx, dz);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] += Ke[(3 * jn + k) * 25] * Ee;
+    }
+    const uint8_t f = fix[n];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (f & (1 << k)) d[k] = 1.0;  // sum over touching elements of 1/nAdj
+  } else {
+    d[0] = d[1] = d[2] = 1.0;  // never used: vectors are 0 off the domain
+  }
+  diag[n] = d[0];
+  diag[g.NN + n] = d[1];
+  diag[2 * g.NN + n] = d[2];
+}
+
+void launch_diag_fine(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                      const uint8_t* nmask, double* diag) {
+  hipLaunchKernelGGL(k_diag_fine, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, Ke, E, fix, nmask, diag);
+}
+
+__global__ void k_diag_coarse(Geo g, const double* __restrict__ Ks, const uint8_t* __restrict__ emask,
+                              const uint8_t* __restrict__ nmask, double* __restrict__ diag) {
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.NN) return;
+  double d[3] = {0.0, 0.0, 0.0};
+  if (nmask[n]) {
+    int iy, ix, iz;
+    node_yxz(n, g, iy, ix, iz);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
+      const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
+      if (!cell_in(g, cy, cx, cz)) continue;
+      const long long ce = nidx(g, cy, cx, cz);
+      if (!emask[ce]) continue;
+      const int jn = loc_index(dy, dx, dz);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] += Ks[(size_t)tri_index(3 * jn + k, 3 * jn + k) * g.NN + ce];
+    }
+  } else {
+    d[0] = d[1] = d[2] = 1.0;
+  }
+  diag[n] = d[0];
+  diag[g.NN + n] = d[1];
+  diag[2 * g.NN + n] = d[2];
+}
+
+void launch_diag_coarse(hipStream_t s, const Geo& g, const double* Ks, const uint8_t* emask,
+                        const uint8_t* nmask, double* diag) {
+  hipLaunchKernelGGL(k_diag_coarse, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, Ks, emask, nmask, diag);
+}
+
+// =========================================================================
+// Galerkin cell matrices (926-996).
+//
+// Level 2 (span 4, no BC child): Ks_c = sum_ss E_ss T_ss, T_ss = Q_ss^T Ke Q_ss
+// precomputed on the host (64 x 300 upper-triangle entries, 950-954, 972-976).
+// =========================================================================
+constexpr int kL2Cells = 16;  // coarse cells per block
+
+__global__ __launch_bounds__(kBlock) void k_galerkin_l2(Geo gf, Geo gc, int span, const double* __restrict__ T,
+                                                        const double* __restrict__ E,
+                                                        const uint8_t* __restrict__ emaskC,
+                                                        const uint8_t* __restrict__ bcFlagC, double* __restrict__ Ks) {
+  const int s3 = span * span * span;  // 64
+  __shared__ double shE[kL2Cells][64];
+  const long long c0 = (long long)blockIdx.x * kL2Cells;
+  for (int t = threadIdx.x; t < kL2Cells * s3; t += kBlock) {
+    const int ci = t / s3, ss = t % s3;
+    const long long cc = c0 + ci;
+    double v = 0.0;
+    if (cc < gc.NN) {
+      int cy, cx, cz;
+      node_yxz(cc, gc, cy, cx, cz);
+      if (cy < gc.ny && cx < gc.nx && cz < gc.nz) {
+        const int ly = ss % span, lx = (ss / span) % span, lz = ss / (span * span);
+        v = E[nidx(gf, span * cy + ly, span * cx + lx, span * cz + lz)];
+      }
+    }
+    shE[ci][ss] = v;
+  }
+  __syncthreads();
+  const int ci = threadIdx.x / 16, chunk = threadIdx.x % 16;
+  const long long cc = c0 + ci;
+  if (cc >= gc.NN || !emaskC[cc] || bcFlagC[cc]) return;
+  const int k0 = chunk * 19, k1 = min(kTri, k0 + 19);
+  double acc[19];
+#pragma unroll
+  for (int i = 0; i < 19; ++i) acc[i] = 0.0;
+  for (int ss = 0; ss < s3; ++ss) {
+    const double e = shE[ci][ss];
+    const double* Tr = T + (size_t)ss * kTri + k0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i)
+      if (k0 + i < k1) acc[i] += Tr[i] * e;
+  }
+#pragma unroll
+  for (int i = 0; i < 19; ++i)
+    if (k0 + i < k1) Ks[(size_t)(k0 + i) * gc.NN + cc] = acc[i];
+}
+
+void launch_galerkin_l2(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* T, const double* E,
+                        const uint8_t* emaskC, const uint8_t* bcFlagC, double* Ks) {
+  const int nb = (int)((gc.NN + kL2Cells - 1) / kL2Cells);
+  hipLaunchKernelGGL(k_galerkin_l2, dim3(nb), dim3(kBlock), 0, s, gf, gc, span, T, E, emaskC, bcFlagC, Ks);
+}
+
+// Interpolation weight of coarse local node J at fine child-local node a for
+// child (ly,lx,lz) of a span-s cell: tensor product of 1-D hat weights.
+__device__ __forceinline__ double hat_w(int span, int ly, int lx, int lz, int a, int J) {
+  const int ty = ly + kLocDy[a], tx = lx + kLocDx[a], tz = lz + kLocDz[a];
+  const double inv = 1.0 / span;
+  const double wy = kLocDy[J] ? ty * inv : (span - ty) * inv;
+  const double wx = kLocDx[J] ? tx * inv : (span - tx) * inv;
+  const double wz = kLocDz[J] ? tz * inv : (span - tz) * inv;
+  return wy * wx * wz;
+}
+
+// Accumulates R += Q^T Kc Q for one child held in shK (24x24 full, row-major)
+// using the Kronecker structure Q = N (x) I3.
+__device__ __forceinline__ void galerkin_child(int span, int ly, int lx, int lz, const double (*shK)[24],
+                                               double (*shT)[24], double* acc, int nThreads) {
+  // shT[a'][(3J+c')] = sum_b K[a'][3b+c'] N[b][J]
+  for (int t = threadIdx.x; t < 576; t += nThreads) {
+    const int row = t / 24, col = t % 24, J = col / 3, cp = col % 3;
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) s += shK[row][3 * b + cp] * hat_w(span, ly, lx, lz, b, J);
+    shT[row][col] = s;
+  }
+  __syncthreads();
+  int i = 0;
+  for (int t = threadIdx.x; t < kTri; t += nThreads, ++i) {
+    int q = 0;
+    while ((q + 1) * (q + 2) / 2 <= t) ++q;
+    const int p = t - q * (q + 1) / 2;  // p <= q
+    const int I = p / 3, c = p % 3;
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) s += hat_w(span, ly, lx, lz, a, I) * shT[3 * a + c][q];
+    acc[i] += s;
+  }
+  __syncthreads();
+}
+
+// Level 2 cells with a child that touches a fixed node (961-971): children get
+// E * P_free Ke P_free + diag(1/nAdj) on their fixed DOFs.
+__global__ __launch_bounds__(kBlock) void k_galerkin_l2_bc(Geo gf, Geo gc, int span, const int* __restrict__ bcCells,
+                                                           const double* __restrict__ Ke, const double* __restrict__ E,
+                                                           const uint8_t* __restrict__ emaskF,
+                                                           const uint8_t* __restrict__ fix, double* __restrict__ Ks) {
+  __shared__ double shK[24][24];
+  __shared__ double shT[24][24];
+  __shared__ uint8_t shF[24];
+  __shared__ double shAdj[8];
+  const long long cc = bcCells[blockIdx.x];
+  int cy, cx, cz;
+  node_yxz(cc, gc, cy, cx, cz);
+  double acc[2] = {0.0, 0.0};
+  const int s3 = span * span * span;
+  for (int ss = 0; ss < s3; ++ss) {
+    const int ly = ss % span, lx = (ss / span) % span, lz = ss / (span * span);
+    const int fy = span * cy + ly, fx = span * cx + lx, fz = span * cz + lz;
+    const long long fe = nidx(gf, fy, fx, fz);
+    const double Ee = E[fe];
+    if (!emaskF[fe]) continue;  // void child contributes nothing
+    if (threadIdx.x < 8) {
+      const int j = threadIdx.x;
+      const int ny_ = fy + kLocDy[j], nx_ = fx + kLocDx[j], nz_ = fz + kLocDz[j];
+      const uint8_t f = fix[nidx(gf, ny_, nx_, nz_)];
+      for (int k = 0; k < 3; ++k) shF[3 * j + k] = (f >> k) & 1;
+      int cnt = 0;
+      for (int c = 0; c < 8; ++c) {
+        const int qy = ny_ - (c & 1), qx = nx_ - ((c >> 1) & 1), qz = nz_ - (c >> 2);
+        if (cell_in(gf, qy, qx, qz) && emaskF[nidx(gf, qy, qx, qz)]) ++cnt;
+      }
+      shAdj[j] = (double)cnt;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 576; t += kBlock) {
+      const int p = t / 24, q = t % 24;
+      double v = (shF[p] || shF[q]) ? 0.0 : Ke[p * 24 + q] * Ee;
+      if (p == q && shF[p]) v = v + 1.0 / shAdj[p / 3];
+      shK[p][q] = v;
+    }
+    __syncthreads();
+    galerkin_child(span, ly, lx, lz, shK, shT, acc, kBlock);
+  }
+  int i = 0;
+  for (int t = threadIdx.x; t < kTri; t += kBlock, ++i) Ks[(size_t)t * gc.NN + cc] = acc[i];
+}
+
+void launch_galerkin_l2_bc(hipStream_t s, const Geo& gf, const Geo& gc, int span, int nBc, const int* bcCells,
+                           const double* Ke, const double* E, const uint8_t* emaskF, const uint8_t* fix,
+                           double* Ks) {
+  if (nBc <= 0) return;
+  hipLaunchKernelGGL(k_galerkin_l2_bc, dim3(nBc), dim3(kBlock), 0, s, gf, gc, span, bcCells, Ke, E, emaskF, fix, Ks);
+}
+
+// Level >= 3 (span 2): Ks_c = sum over 8 children of Q^T Ks_child Q (980-995).
+__global__ __launch_bounds__(64) void k_galerkin_coarse(Geo gf, Geo gc, const double* __restrict__ KsF,
+                                                        const uint8_t* __restrict__ emaskF,
+                                                        const uint8_t* __restrict__ emaskC, double* __restrict__ KsC) {
+  __shared__ double shK[24][24];
+  __shared__ double shT[24][24];
+  const long long cc = blockIdx.x;
+  if (cc >= gc.NN || !emaskC[cc]) return;
+  int cy, cx, cz;
+  node_yxz(cc, gc, cy, cx, cz);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int ch = 0; ch < 8; ++ch) {
+    const int ly = ch & 1, lx = (ch >> 1) & 1, lz = ch >> 2;
+    const long long fe = nidx(gf, 2 * cy + ly, 2 * cx + lx, 2 * cz + lz);
+    if (!emaskF[fe]) continue;
+    for (int t = threadIdx.x; t < 576; t += 64) {
+      const int p = t / 24, q = t % 24;
+      shK[p][q] = KsF[(size_t)tri_index(p, q) * gf.NN + fe];
+    }
+    __syncthreads();
+    galerkin_child(2, ly, lx, lz, shK, shT, acc, 64);
+  }
+  int i = 0;
+  for (int t = threadIdx.x; t < kTri; t += 64, ++i) KsC[(size_t)t * gc.NN + cc] = acc[i];
+}
+
+void launch_galerkin_coarse(hipStream_t s, const Geo& gf, const Geo& gc, const double* KsF, const uint8_t* emaskF,
+                            const uint8_t* emaskC, double* KsC) {
+  hipLaunchKernelGGL(k_galerkin_coarse, dim3((unsigned)gc.NN), dim3(64), 0, s, gf, gc, KsF, emaskF, emaskC, KsC);
+}
+
+// =========================================================================
+// Multigrid transfer (783-814) as masked tensor-product stencils.
+// R = P^T with P the trilinear hat interpolation; the reference's division
+// by transferMatCoeffi and solid-node selection reduce to exactly this.
+// =========================================================================
+__global__ __launch_bounds__(kBlock) void k_restrict(Geo gf, Geo gc, int span, const double* __restrict__ rf,
+                                                     const uint8_t* __restrict__ nmaskC, double* __restrict__ rc,
+                                                     const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= gc.NN) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  if (nmaskC[n]) {
+    int Jy, Jx, Jz;
+    node_yxz(n, gc, Jy, Jx, Jz);
+    const int y0 = span * Jy, x0 = span * Jx, z0 = span * Jz;
+    const double inv = 1.0 / span;
+    for (int dz = -(span - 1); dz <= span - 1; ++dz) {
+      const int z = z0 + dz;
+      if (z < 0 || z >= gf.NZ) continue;
+      const double wz = (span - abs(dz)) * inv;
+      for (int dx = -(span - 1); dx <= span - 1; ++dx) {
+        const int x = x0 + dx;
+        if (x < 0 || x >= gf.NX) continue;
+        const double wzx = wz * ((span - abs(dx)) * inv);
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int dy = -(span - 1); dy <= span - 1; ++dy) {
+          const int y = y0 + dy;
+          if (y < 0 || y >= gf.NY) continue;
+          const double wy = (span - abs(dy)) * inv;
+          const long long f = nidx(gf, y, x, z);
+          t0 += wy * rf[f];
+          t1 += wy * rf[gf.NN + f];
+          t2 += wy * rf[2 * gf.NN + f];
+        }
+        a0 += wzx * t0;
+        a1 += wzx * t1;
+        a2 += wzx * t2;
+      }
+    }
+  }
+  rc[n] = a0;
+  rc[gc.NN + n] = a1;
+  rc[2 * gc.NN + n] = a2;
+}
+
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, const uint8_t* nmaskC,
+                     double* rc, const int* fl) {
+  hipLaunchKernelGGL(k_restrict, dim3(grid_for(gc.NN)), dim3(kBlock), 0, s, gf, gc, span, rf, nmaskC, rc, fl);
+}
+
+// out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere
+// (758, 770, 773).  Optionally fuses the z.r reduction of the PCG (629).
+__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(Geo gf, Geo gc, int span, double omega,
+                                                           const double* __restrict__ xc,
+                                                           const double* __restrict__ rf,
+                                                           const double* __restrict__ diag,
+                                                           const uint8_t* __restrict__ nmaskF,
+                                                           double* __restrict__ out, RedWs ws, double* sc,
+                                                           const int* fl, int mode) {
+  if (fl && fl[FL_DONE]) return;
+  const long long NN = gf.NN;
+  double zr = 0.0;
+  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < NN; n += (long long)gridDim.x * kBlock) {
+    double o[3] = {0.0, 0.0, 0.0};
+    if (nmaskF[n]) {
+      int y, x, z;
+      node_yxz(n, gf, y, x, z);
+      const int Jy = y / span, Jx = x / span, Jz = z / span;
+      const int ty = y - span * Jy, tx = x - span * Jx, tz = z - span * Jz;
+      const double inv = 1.0 / span;
+      const double wy[2] = {(span - ty) * inv, ty * inv};
+      const double wx[2] = {(span - tx) * inv, tx * inv};
+      const double wz[2] = {(span - tz) * inv, tz * inv};
+      double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
+        const double w = wy[a] * wx[b] * wz[d];
+        if (w == 0.0) continue;
+        const long long J = nidx(gc, Jy + a, Jx + b, Jz + d);
+        p[0] += w * xc[J];
+        p[1] += w * xc[gc.NN + J];
+        p[2] += w * xc[2 * gc.NN + J];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double r = rf[k * NN + n], dd = diag[k * NN + n];
+        o[k] = (omega * (r / dd) + p[k]) + (omega * r) / dd;
+        zr += o[k] * r;
+      }
+    }
+    out[n] = o[0];
+    out[NN + n] = o[1];
+    out[2 * NN + n] = o[2];
+  }
+  if (mode > 0) {
+    double v[1] = {zr};
+    write_partials<1>(v, ws.partials);
+  }
+}
+
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
+                           const double* rf, const double* diag, const uint8_t* nmaskF, double* out,
+                           const RedWs* ws, double* sc, const int* fl) {
+  // ws != nullptr: fused reduction; mode 1 = initial z.r, mode 2 = in-loop
+  int mode = 0;
+  RedWs w{nullptr, nullptr};
+  if (ws) { w = *ws; mode = (fl && sc) ? 2 : 1; }
+  const int nb = mode ? red_blocks(gf.NN) : grid_for(gf.NN);
+  hipLaunchKernelGGL(k_prolong_jacobi, dim3(nb), dim3(kBlock), 0, s, gf, gc, span, omega, xc, rf, diag, nmaskF, out,
+                     w, sc, fl, mode);
+  if (mode) fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, nb, w, sc, (int*)fl, nullptr);
+}
+
+// =========================================================================
+// Coarsest level: banded Cholesky (1052-1065) and solves (767).
+// Band storage: row i holds L(i, i-k) at band[i*W + k], W = bw+1.
+// =========================================================================
+__global__ void k_coarse_gather(int n, const long long* __restrict__ map, const double* __restrict__ r,
+                                double* __restrict__ v, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) v[i] = r[map[i]];
+}
+void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl) {
+  hipLaunchKernelGGL(k_coarse_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, r, v, fl);
+}
+__global__ void k_coarse_scatter(int n, const long long* __restrict__ map, const double* __restrict__ v,
+                                 double* __restrict__ x, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[map[i]] = v[i];
+}
+void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long,
+                           const int* fl) {
+  hipLaunchKernelGGL(k_coarse_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, v, x, fl);
+}
+
+// Node-centric assembly of the lower band: the thread of coarsest node n owns
+// rows 3*pos(n)+c, so no two threads write the same entry.
+__global__ void k_band_assemble(Geo g, const double* __restrict__ Ks, const uint8_t* __restrict__ emask,
+                                const int* __restrict__ nodePos, int bw, double* __restrict__ band) {
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.NN) return;
+  const int pn = nodePos[n];
+  if (pn < 0) return;
+  const int W = bw + 1;
+  int iy, ix, iz;
+  node_yxz(n, g, iy, ix, iz);
+  for (int c = 0; c < 8; ++c) {
+    const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
+    const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
+    if (!cell_in(g, cy, cx, cz)) continue;
+    const long long ce = nidx(g, cy, cx, cz);
+    if (!emask[ce]) continue;
+    const int jn = loc_index(dy, dx, dz);
+    for (int b = 0; b < 8; ++b) {
+      const int pb = nodePos[nidx(g, cy + kLocDy[b], cx + kLocDx[b], cz + kLocDz[b])];
+      for (int k = 0; k < 3; ++k)
+        for (int kk = 0; kk < 3; ++kk) {
+          const long long row = 3LL * pn + k, col = 3LL * pb + kk;
+          if (col > row) continue;
+          band[row * W + (row - col)] += Ks[(size_t)tri_index(3 * jn + k, 3 * b + kk) * g.NN + ce];
+        }
+    }
+  }
+}
+void launch_band_assemble(hipStream_t s, const Geo& g, const double* Ks, const uint8_t* emask, const int* nodePos,
+                          int bw, double* band, long long nBand) {
+  launch_fill(s, band, 0.0, nBand);
+  hipLaunchKernelGGL(k_band_assemble, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, Ks, emask, nodePos, bw, band);
+}
+
+// Right-looking banded Cholesky in one work-group.
+constexpr int kFactBlock = 1024;
+__global__ __launch_bounds__(kFactBlock) void k_band_factor(int n, int bw, double* __restrict__ B, int* status) {
+  const int W = bw + 1;
+  __shared__ double shd;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (threadIdx.x == 0) {
+      const double a = B[(size_t)j * W];
+      if (!(a > 0.0)) bad = 1;
+      shd = sqrt(a);
+      B[(size_t)j * W] = shd;
+    }
+    __syncthreads();
+    if (bad) break;
+    const int iend = min(n - 1, j + bw);
+    const int m = iend - j;
+    const double d = shd;
+    for (int i = j + 1 + threadIdx.x; i <= iend; i += kFactBlock) B[(size_t)i * W + (i - j)] /= d;
+    __syncthreads();
+    const long long tot = (long long)m * (m + 1) / 2;
+    for (long long t = threadIdx.x; t < tot; t += kFactBlock) {
+      int ii = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+      while ((long long)(ii + 1) * (ii + 2) / 2 <= t) ++ii;
+      while ((long long)ii * (ii + 1) / 2 > t) --ii;
+      const int kk = (int)(t - (long long)ii * (ii + 1) / 2);
+      const int i = j + 1 + ii, k = j + 1 + kk;  // k <= i
+      B[(size_t)i * W + (i - k)] -= B[(size_t)i * W + (i - j)] * B[(size_t)k * W + (k - j)];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *status = bad;
+}
+int launch_band_factor(hipStream_t s, int n, int bw, double* band, int* status) {
+  hipLaunchKernelGGL(k_band_factor, dim3(1), dim3(kFactBlock), 0, s, n, bw, band, status);
+  return 0;
+}
+
+// Transposed band (bandT[i*W+k] = L(i+k, i)) and inverses of the nb x nb
+// diagonal blocks (row-major lower triangles), one block per work-group.
+__global__ void k_band_transpose(int n, int bw, const double* __restrict__ B, double* __restrict__ BT) {
+  const int W = bw + 1;
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)n * W) return;
+  const int i = (int)(t / W), k = (int)(t % W);
+  BT[t] = (i + k < n) ? B[(size_t)(i + k) * W + k] : 0.0;
+}
+__global__ void k_band_dinv(int n, int bw, int nb, const double* __restrict__ B, double* __restrict__ Dinv) {
+  const int W = bw + 1;
+  const int r0 = blockIdx.x * nb;
+  const int m = min(nb, n - r0);
+  double* D = Dinv + (size_t)blockIdx.x * nb * nb;
+  const int c = threadIdx.x;  // column of the inverse
+  if (c >= nb) return;
+  for (int i = 0; i < nb; ++i) D[(size_t)i * nb + c] = 0.0;
+  if (c >= m) return;
+  auto L = [&](int i, int k) { return (i - k <= bw) ? B[(size_t)(r0 + i) * W + (i - k)] : 0.0; };
+  D[(size_t)c * nb + c] = 1.0 / L(c, c);
+  for (int i = c + 1; i < m; ++i) {
+    double s = 0.0;
+    for (int k = c; k < i; ++k) s += L(i, k) * D[(size_t)k * nb + c];
+    D[(size_t)i * nb + c] = -s / L(i, i);
+  }
+}
+void launch_band_post(hipStream_t s, int n, int bw, int nb, const double* band, double* bandT, double* dinv) {
+  const long long tot = (long long)n * (bw + 1);
+  hipLaunchKernelGGL(k_band_transpose, dim3(grid_for(tot)), dim3(kBlock), 0, s, n, bw, band, bandT);
+  const int nblk = (n + nb - 1) / nb;
+  hipLaunchKernelGGL(k_band_dinv, dim3(nblk), dim3(nb), 0, s, n, bw, nb, band, dinv);
+}
+
+// Forward and backward substitution in one work-group, nb = 64 rows per step:
+// the off-block part is a GEMV over the band, the block part a GEMV with the
+// precomputed diagonal-block inverse.
+constexpr int kSolveNb = 64;
+__global__ __launch_bounds__(kBlock) void k_band_solve(int n, int bw, const double* __restrict__ B,
+                                                       const double* __restrict__ BT,
+                                                       const double* __restrict__ Dinv, double* __restrict__ v,
+                                                       const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int W = bw + 1, nb = kSolveNb;
+  __shared__ double sh[kSolveNb];
+  const int nblk = (n + nb - 1) / nb;
+  const int row = threadIdx.x >> 2, lane = threadIdx.x & 3;
+  // forward: L y = v
+  for (int b = 0; b < nblk; ++b) {
+    const int r0 = b * nb, m = min(nb, n - r0);
+    if (row < m) {
+      const int i = r0 + row;
+      const int k0 = max(0, i - bw);
+      double s = 0.0;
+      for (int k = k0 + lane; k < r0; k += 4) s += B[(size_t)i * W + (i - k)] * v[k];
+      s += __shfl_xor(s, 1, 64);
+      s += __shfl_xor(s, 2, 64);
+      if (lane == 0) sh[row] = v[i] - s;
+    }
+    __syncthreads();
+    if (threadIdx.x < m) {
+      const double* D = Dinv + (size_t)b * nb * nb + (size_t)threadIdx.x * nb;
+      double y = 0.0;
+      for (int j = 0; j <= (int)threadIdx.x; ++j) y += D[j] * sh[j];
+      v[r0 + threadIdx.x] = y;
+    }
+    __syncthreads();
+  }
+  // backward: L^T x = y
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int r0 = b * nb, m = min(nb, n - r0), r1 = r0 + m;
+    if (row < m) {
+      const int i = r0 + row;
+      const int k1 = min(n - 1, i + bw);
+      double s = 0.0;
+      for (int k = r1 + lane; k <= k1; k += 4) s += BT[(size_t)i * W + (k - i)] * v[k];
+      s += __shfl_xor(s, 1, 64);
+      s += __shfl_xor(s, 2, 64);
+      if (lane == 0) sh[row] = v[i] - s;
+    }
+    __syncthreads();
+    if (threadIdx.x < m) {
+      const double* D = Dinv + (size_t)b * nb * nb;
+      double x = 0.0;
+      for (int j = threadIdx.x; j < m; ++j) x += D[(size_t)j * nb + threadIdx.x] * sh[j];
+      v[r0 + threadIdx.x] = x;
+    }
+    __syncthreads();
+  }
+}
+void launch_band_solve(hipStream_t s, int n, int bw, int nb, const double* band, const double* bandT,
+                       const double* dinv, double* v, const int* fl) {
+  (void)nb;
+  hipLaunchKernelGGL(k_band_solve, dim3(1), dim3(kBlock), 0, s, n, bw, band, bandT, dinv, v, fl);
+}
+
+// =========================================================================
+// Vector kernels of the PCG (600-639).
+// =========================================================================
+__global__ void k_fill(double* p, double v, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) p[i] = v;
+}
+void launch_fill(hipStream_t s, double* p, double v, long long n) {
+  hipLaunchKernelGGL(k_fill, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, p, v, n);
+}
+__global__ void k_copy(double* d, const double* a, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) d[i] = a[i];
+}
+void launch_copy(hipStream_t s, double* dst, const double* src, long long n) {
+  hipLaunchKernelGGL(k_copy, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
+}
+__global__ void k_sub(double* r, const double* b, const double* q, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    r[i] = b[i] - q[i];
+}
+void launch_sub(hipStream_t s, double* r, const double* b, const double* q, long long n) {
+  hipLaunchKernelGGL(k_sub, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, b, q, n);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a, const double* __restrict__ b,
+                                                long long n, RedWs ws, double* out, int sq) {
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    s += a[i] * b[i];
+  double v[1] = {s};
+  write_partials<1>(v, ws.partials);
+  (void)out;
+  (void)sq;
+}
+void launch_norm2(hipStream_t s, const double* v, long long n, const RedWs& ws, double* out) {
+  hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, s, v, v, n, ws, out, 1);
+  fin1(s, FIN_SQRT, red_blocks(n), ws, nullptr, nullptr, out);
+}
+void launch_dot(hipStream_t s, const double* a, const double* b, long long n, const RedWs& ws, double* out) {
+  hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, s, a, b, n, ws, out, 0);
+  fin1(s, FIN_SUM, red_blocks(n), ws, nullptr, nullptr, out);
+}
+
+// y += lambda p ; r -= lambda q ; resnorm = ||r|| / ||b|| ; convergence flag.
+__global__ __launch_bounds__(kBlock) void k_pcg_update(double* __restrict__ y, double* __restrict__ r,
+                                                       const double* __restrict__ p, const double* __restrict__ q,
+                                                       long long n, RedWs ws, double* sc, int* fl) {
+  if (fl[FL_DONE]) return;
+  const double lambda = sc[SC_LAMBDA];
+  double rr = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    y[i] = y[i] + lambda * p[i];                // 616
+    const double ri = r[i] - lambda * q[i];     // 617
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  double v[1] = {rr};
+  write_partials<1>(v, ws.partials);
+}
+void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, const double* q, long long n,
+                       const RedWs& ws, double* sc, int* fl) {
+  hipLaunchKernelGGL(k_pcg_update, dim3(red_blocks(n)), dim3(kBlock), 0, s, y, r, p, q, n, ws, sc, fl);
+  fin1(s, FIN_RR, red_blocks(n), ws, sc, fl, nullptr);
+}
+
+__global__ void k_pcg_pupdate(double* __restrict__ p, const double* __restrict__ z, long long n,
+                              const double* sc, const int* fl) {
+  if (fl[FL_DONE]) return;
+  const double beta = sc[SC_BETA];
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    p[i] = z[i] + beta * p[i];  // 630
+}
+void launch_pcg_pupdate(hipStream_t s, double* p, const double* z, long long n, const double* sc, const int* fl) {
+  hipLaunchKernelGGL(k_pcg_pupdate, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, p, z, n, sc, fl);
+}
+
+__global__ void k_pcg_init(double* sc, int* fl, double tol) {
+  sc[SC_TOL] = tol;
+  fl[FL_DONE] = 0;
+  fl[FL_ITS] = 0;
+  fl[FL_ITER] = 0;
+}
+void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol) {
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1), 0, s, sc, fl, tol);
+}
+
+// z = dinv .* r (889) with the fused z.r reduction.
+__global__ __launch_bounds__(kBlock) void k_scale_diag(double* __restrict__ z, const double* __restrict__ dinv,
+                                                       const double* __restrict__ r, long long n, RedWs ws,
+                                                       double* sc, const int* fl, int mode) {
+  if (fl && fl[FL_DONE]) return;
+  double zr = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    const double zi = dinv[i] * r[i];
+    z[i] = zi;
+    zr += zi * r[i];
+  }
+  double v[1] = {zr};
+  write_partials<1>(v, ws.partials);
+  (void)mode;
+}
+void launch_scale_diag(hipStream_t s, double* z, const double* dinv, const double* r, long long n,
+                       const RedWs* ws, double* sc, const int* fl) {
+  const int mode = fl ? 2 : 1;
+  hipLaunchKernelGGL(k_scale_diag, dim3(red_blocks(n)), dim3(kBlock), 0, s, z, dinv, r, n, *ws, sc, fl, mode);
+  fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, red_blocks(n), *ws, sc, (int*)fl, nullptr);
+}
+
+__global__ void k_gather_compact(const int* __restrict__ map, int n, int ncomp, long long NN,
+                                 const double* __restrict__ dense, double* __restrict__ compact) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)n * ncomp) return;
+  const int i = (int)(t / ncomp), c = (int)(t % ncomp);
+  compact[t] = dense[(size_t)c * NN + map[i]];
+}
+void launch_gather_compact(hipStream_t s, const int* map, int n, int ncomp, long long NN, const double* dense,
+                           double* compact) {
+  hipLaunchKernelGGL(k_gather_compact, dim3(grid_for((long long)n * ncomp)), dim3(kBlock), 0, s, map, n, ncomp, NN,
+                     dense, compact);
+}
+__global__ void k_scatter_compact(const int* __restrict__ map, int n, int ncomp, long long NN,
+                                  const double* __restrict__ compact, double* __restrict__ dense) {
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (long long)n * ncomp) return;
+  const int i = (int)(t / ncomp), c = (int)(t % ncomp);
+  dense[(size_t)c * NN + map[i]] = compact[t];
+}
+void launch_scatter_compact(hipStream_t s, const int* map, int n, int ncomp, long long NN, const double* compact,
+                            double* dense) {
+  hipLaunchKernelGGL(k_scatter_compact, dim3(grid_for((long long)n * ncomp)), dim3(kBlock), 0, s, map, n, ncomp, NN,
+                     compact, dense);
+}
+
+// =========================================================================
+// PDE filter (816-917): scalar 8x8 Helmholtz kernel on the node grid.
+// =========================================================================
+__global__ void k_filter_e2n(Geo g, const uint8_t* __restrict__ emask, const double* __restrict__ src,
+                             double* __restrict__ b) {
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.NN) return;
+  int iy, ix, iz;
+  node_yxz(n, g, iy, ix, iz);
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int cy = iy - (c & 1), cx = ix - ((c >> 1) & 1), cz = iz - (c >> 2);
+    if (!cell_in(g, cy, cx, cz)) continue;
+    const long long ce = nidx(g, cy, cx, cz);
+    if (emask[ce]) s += src[ce] * (1.0 / 8.0);  // 881-885
+  }
+  b[n] = s;
+}
+void launch_filter_e2n(hipStream_t s, const Geo& g, const uint8_t* emask, const double* src, double* b) {
+  hipLaunchKernelGGL(k_filter_e2n, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, emask, src, b);
+}
+
+__global__ __launch_bounds__(kBlock) void k_filter_apply(Geo g, const double* __restrict__ kern,
+                                                         const uint8_t* __restrict__ emask,
+                                                         const double* __restrict__ u, double* __restrict__ y,
+                                                         RedWs ws, double* sc, const int* fl, int pcgStep) {
+  if (fl && fl[FL_DONE]) return;
+  double pq = 0.0;
+  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < g.NN; n += (long long)gridDim.x * kBlock) {
+    int iy, ix, iz;
+    node_yxz(n, g, iy, ix, iz);
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
+      const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
+      if (!cell_in(g, cy, cx, cz)) continue;
+      if (!emask[nidx(g, cy, cx, cz)]) continue;
+      const int jn = loc_index(dy, dx, dz);
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) s += u[nidx(g, cy + kLocDy[b], cx + kLocDx[b], cz + kLocDz[b])] * kern[jn * 8 + b];
+      acc += s;
+    }
+    y[n] = acc;
+    pq += u[n] * acc;
+  }
+  if (pcgStep) {
+    double v[1] = {pq};
+    write_partials<1>(v, ws.partials);
+  }
+}
+void launch_filter_apply(hipStream_t s, const Geo& g, const double* kern, const uint8_t* emask, const double* u,
+                         double* y, const RedWs* ws, double* sc, const int* fl) {
+  RedWs w{nullptr, nullptr};
+  int step = 0;
+  if (ws) { w = *ws; step = 1; }
+  const int nb = step ? red_blocks(g.NN) : grid_for(g.NN);
+  hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(kBlock), 0, s, g, kern, emask, u, y, w, sc, fl, step);
+  if (step) fin1(s, FIN_PQ, nb, w, sc, (int*)fl, nullptr);
+}
+
+__global__ void k_filter_n2e(Geo g, const uint8_t* __restrict__ emask, const double* __restrict__ u,
+                             double* __restrict__ tar) {
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.NN) return;
+  if (!emask[n]) { tar[n] = 0.0; return; }
+  int iy, ix, iz;
+  node_yxz(n, g, iy, ix, iz);
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += u[nidx(g, iy + kLocDy[j], ix + kLocDx[j], iz + kLocDz[j])];
+  tar[n] = s * (1.0 / 8.0);  // 899-901
+}
+void launch_filter_n2e(hipStream_t s, const Geo& g, const uint8_t* emask, const double* u, double* tar) {
+  hipLaunchKernelGGL(k_filter_n2e, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, emask, u, tar);
+}
+
+__global__ void k_filter_diag(Geo g, const double* __restrict__ kern, const uint8_t* __restrict__ emask,
+                              const uint8_t* __restrict__ nmask, double* __restrict__ dinv) {
+  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (n >= g.NN) return;
+  if (!nmask[n]) { dinv[n] = 0.0; return; }
+  int iy, ix, iz;
+  node_yxz(n, g, iy, ix, iz);
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
+    const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
+    if (!cell_in(g, cy, cx, cz) || !emask[nidx(g, cy, cx, cz)]) continue;
+    s += kern[loc_index(dy, dx, dz) * 9];
+  }
+  dinv[n] = 1.0 / s;  // 867
+}
+void launch_filter_diag(hipStream_t s, const Geo& g, const double* kern, const uint8_t* emask,
+                        const uint8_t* nmask, double* dinv) {
+  hipLaunchKernelGGL(k_filter_diag, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, kern, emask, nmask, dinv);
+}
+
+// =========================================================================
+// Optimiser kernels.
+// =========================================================================
+__global__ void k_simp(long long n, const uint8_t* __restrict__ emask, const double* __restrict__ xPhys,
+                       double Emin, double E0, double* __restrict__ E) {
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double x = xPhys[i];
+  E[i] = emask[i] ? Emin + (x * x * x) * (E0 - Emin) : 0.0;  // 107
+}
+void launch_simp(hipStream_t s, long long n, const uint8_t* emask, const double* xPhys, double Emin, double E0,
+                 double* E) {
+  hipLaunchKernelGGL(k_simp, dim3(grid_for(n)), dim3(kBlock), 0, s, n, emask, xPhys, Emin, E0, E);
+}
+
+// Masked sums over cells: mode 0: sum a; 1: sum a*b; 2: sum a*(1-a);
+// 3: sum a^16 / b^16 (p-norm, 335/360).
+__global__ __launch_bounds__(kBlock) void k_masked_sum(long long n, const uint8_t* __restrict__ emask,
+                                                       const double* __restrict__ a, const double* __restrict__ b,
+                                                       int mode, RedWs ws, double* out) {
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (!emask[i]) continue;
+    const double ai = a[i];
+    switch (mode) {
+      case 0: s += ai; break;
+      case 1: s += ai * b[i]; break;
+      case 2: s += ai * (1.0 - ai); break;
+      case 3: {
+        const double a2 = ai * ai, a4 = a2 * a2, a8 = a4 * a4;
+        const double bi = b[i], b2 = bi * bi, b4 = b2 * b2, b8 = b4 * b4;
+        s += (a8 * a8) / (b8 * b8);
+      } break;
+      default: break;
+    }
+  }
+  double v[1] = {s};
+  write_partials<1>(v, ws.partials);
+  (void)out;
+}
+void launch_masked_sum(hipStream_t s, long long n, const uint8_t* emask, const double* a, const double* b, int mode,
+                       const RedWs& ws, double* out) {
+  hipLaunchKernelGGL(k_masked_sum, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, a, b, mode, ws, out);
+  fin1(s, FIN_SUM, red_blocks(n), ws, nullptr, nullptr, out);
+}
+
+// ---- OC bisection (145-153), k-ary: one pass evaluates the 2^depth-1
+// midpoints of the next `depth` bisection levels; the last block replays the
+// sequential decisions with identical arithmetic.
+// oc[]: 0 l1, 1 l2, 2 fval, 3 done, 4 lastLmid, 5 nBisect, 6 change
+constexpr int kOcDepth = 3;
+constexpr int kOcCand = (1 << kOcDepth) - 1;
+
+__device__ __forceinline__ double oc_xnew(double x, double dc, double lmid) {
+  const double move = 0.2;
+  double v = x * sqrt(-dc / 1.0 / lmid);  // 149
+  v = fmin(x + move, v);
+  v = fmin(1.0, v);
+  v = fmax(x - move, v);
+  return fmax(0.0, v);
+}
+
+__global__ __launch_bounds__(kBlock) void k_oc_pass(long long n, const uint8_t* __restrict__ emask,
+                                                    const uint8_t* __restrict__ passive,
+                                                    const double* __restrict__ x, const double* __restrict__ dc,
+                                                    RedWs ws, double* oc) {
+  if (oc[3] != 0.0) return;
+  const double l1 = oc[0], l2 = oc[1];
+  // heap-ordered candidate intervals: node k -> (a,b); children 2k (gt<=0), 2k+1 (gt>0)
+  double lm[kOcCand + 1], A[2 * kOcCand + 2], Bv[2 * kOcCand + 2];
+  A[1] = l1;
+  Bv[1] = l2;
+  for (int k = 1; k <= kOcCand; ++k) {
+    lm[k] = 0.5 * (Bv[k] + A[k]);
+    if (2 * k + 1 <= 2 * kOcCand + 1) {
+      A[2 * k] = A[k];
+      Bv[2 * k] = lm[k];
+      A[2 * k + 1] = lm[k];
+      Bv[2 * k + 1] = Bv[k];
+    }
+  }
+  double s[kOcCand];
+#pragma unroll
+  for (int k = 0; k < kOcCand; ++k) s[k] = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (!emask[i]) continue;
+    const double xi = x[i], di = dc[i];
+    const bool pas = passive && passive[i];
+#pragma unroll
+    for (int k = 0; k < kOcCand; ++k) {
+      const double xn = pas ? 1.0 : oc_xnew(xi, di, lm[k + 1]);
+      s[k] += 1.0 * (xn - xi);  // 151
+    }
+  }
+  write_partials<kOcCand>(s, ws.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_oc_fin(int nb, const double* __restrict__ partials, double* oc) {
+  if (oc[3] != 0.0) return;
+  double s[kOcCand];
+  fold_partials<kOcCand>(partials, nb, s);
+  if (threadIdx.x != 0) return;
+  const double l1 = oc[0], l2 = oc[1];
+  double a = l1, b = l2;
+  int node = 1;
+  const double fval = oc[2];
+  int nbis = (int)oc[5];
+  double last = oc[4];
+  bool done = false;
+  for (int d = 0; d < kOcDepth; ++d) {
+    if (!((b - a) / (a + b) > 1e-6)) { done = true; break; }  // 147
+    const double lmid = 0.5 * (b + a);                         // 148
+    const double gt = fval + s[node - 1];                      // 151
+    if (gt > 0) { a = lmid; node = 2 * node + 1; } else { b = lmid; node = 2 * node; }  // 152
+    last = lmid;
+    ++nbis;
+  }
+  if (!((b - a) / (a + b) > 1e-6)) done = true;
+  oc[0] = a;
+  oc[1] = b;
+  oc[4] = last;
+  oc[5] = (double)nbis;
+  if (done) oc[3] = 1.0;
+}
+void launch_oc_pass(hipStream_t s, long long n, const uint8_t* emask, const uint8_t* passive, const double* x,
+                    const double* dc, const RedWs& ws, double* oc, int) {
+  hipLaunchKernelGGL(k_oc_pass, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, passive, x, dc, ws, oc);
+  hipLaunchKernelGGL(k_oc_fin, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, oc);
+}
+
+// xnew at the last evaluated lmid (149-150), change = max|xnew-x| (154), x = xnew.
+__global__ __launch_bounds__(kBlock) void k_oc_final(long long n, const uint8_t* __restrict__ emask,
+                                                     const uint8_t* __restrict__ passive, double* __restrict__ x,
+                                                     const double* __restrict__ dc, RedWs ws, double* oc) {
+  const double lmid = oc[4];
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (!emask[i]) continue;
+    const double xi = x[i];
+    const double xn = (passive && passive[i]) ? 1.0 : oc_xnew(xi, dc[i], lmid);
+    m = fmax(m, fabs(xn - xi));
+    x[i] = xn;
+  }
+  write_partial_max(m, ws.partials);
+}
+void launch_oc_final(hipStream_t s, long long n, const uint8_t* emask, const uint8_t* passive, double* x,
+                     const double* dc, const RedWs& ws, double* oc) {
+  hipLaunchKernelGGL(k_oc_final, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, passive, x, dc, ws, oc);
+  hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, oc + 6);
+}
+
+__global__ __launch_bounds__(kBlock) void k_masked_maxdiff(long long n, const uint8_t* __restrict__ emask,
+                                                           const double* __restrict__ a,
+                                                           const double* __restrict__ b, RedWs ws, double* out) {
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    if (emask[i]) m = fmax(m, fabs(a[i] - b[i]));
+  write_partial_max(m, ws.partials);
+  (void)out;
+}
+void launch_masked_maxdiff(hipStream_t s, long long n, const uint8_t* emask, const double* a, const double* b,
+                           const RedWs& ws, double* out) {
+  hipLaunchKernelGGL(k_masked_maxdiff, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, a, b, ws, out);
+  hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, out);
+}
+
+// Elementwise cell operations (masked; non-domain cells are left 0).
+__global__ void k_elementwise(int op, long long n, const uint8_t* __restrict__ emask, double* __restrict__ a,
+                              const double* __restrict__ b, const double* __restrict__ c, double p0, double p1,
+                              double p2) {
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (!emask[i]) { a[i] = 0.0; return; }
+  switch (op) {
+    case OP_DC: {  // a = -(p0 * x^2) * (ce / p1)   (108, 129)
+      const double x = b[i];
+      a[i] = -(p0 * (x * x)) * (c[i] / p1);
+    } break;
+    case OP_MUL: a[i] = b[i] * c[i]; break;
+    case OP_DIVMAX: a[i] = a[i] / fmax(1e-3, b[i]); break;  // 137
+    case OP_HEAVI: {  // 380: p0 = beta, p1 = eta
+      const double beta = p0, eta = p1;
+      a[i] = (tanh(beta * eta) + tanh(beta * (b[i] - eta))) / (tanh(beta * eta) + tanh(beta * (1 - eta)));
+    } break;
+    case OP_DHEAVI: {  // 340
+      const double beta = p0, eta = p1;
+      const double th = tanh(beta * (b[i] - eta));
+      a[i] = beta * (1 - th * th) / (tanh(beta * eta) + tanh(beta * (1 - eta)));
+    } break;
+    case OP_PNORMGRAD: {  // 335: p0 = fac, p1 = numElements
+      const double xh = b[i], x2 = xh * xh, x4 = x2 * x2, x8 = x4 * x4;
+      const double v = c[i], v2 = v * v, v4 = v2 * v2, v8 = v4 * v4;
+      a[i] = p0 * ((x8 * x4 * x2 * xh) / (v8 * v8)) / p1;
+    } break;
+    case OP_COPY: a[i] = b[i]; break;
+    case OP_SET: a[i] = p0; break;
+    case OP_DIVC: a[i] = b[i] / p0; break;
+    case OP_MMA_BOUNDS: {  // a = xmin = max(0, x - move), c-> xmax written by the caller's second call
+      a[i] = p1 > 0.0 ? fmin(1.0, b[i] + p0) : fmax(0.0, b[i] - p0);  // 364-365
+    } break;
+    default: break;
+  }
+}
+void launch_elementwise(hipStream_t s, int op, long long n, const uint8_t* emask, double* a, const double* b,
+                        const double* c, double p0, double p1, double p2) {
+  hipLaunchKernelGGL(k_elementwise, dim3(grid_for(n)), dim3(kBlock), 0, s, op, n, emask, a, b, c, p0, p1, p2);
+}
+
+// =========================================================================
+// MMA for m = 1 (MMAseq, 1804-2049), every dual step on the device.
+// Scalar block mm[]: see MmaSlot.  Arrays are dense cell arrays (emask).
+// =========================================================================
+
+// GenSub (1873-1928) + the xold shift of Update (1867-1868) + SolveDIP init.
+__global__ __launch_bounds__(kBlock) void k_mma_gensub(long long n, const uint8_t* __restrict__ emask, int iter,
+                                                       const double* __restrict__ x, const double* __restrict__ xmin,
+                                                       const double* __restrict__ xmax, double* __restrict__ xold1,
+                                                       double* __restrict__ xold2, double* __restrict__ low,
+                                                       double* __restrict__ upp, const double* __restrict__ dfdx,
+                                                       const double* __restrict__ dgdx, double* __restrict__ alpha,
+                                                       double* __restrict__ beta, double* __restrict__ p0,
+                                                       double* __restrict__ q0, double* __restrict__ pij,
+                                                       double* __restrict__ qij, RedWs ws, double* mm, double nAct) {
+  const double asyinit = 0.5, asydec = 0.7, asyinc = 1.2, feps = 1.0e-6;
+  double bs = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (!emask[i]) continue;
+    const double xv = x[i], mn = xmin[i], mx = xmax[i];
+    const double o1 = xold1[i], o2 = xold2[i];
+    double lo, up;
+    if (iter < 3) {  // 1880-1882
+      lo = xv - asyinit * (mx - mn);
+      up = xv + asyinit * (mx - mn);
+    } else {  // 1884-1900
+      const double zzz = (xv - o1) * (o1 - o2);
+      double gamma = 1.0;
+      if (zzz > 0) gamma = asyinc;
+      if (zzz < 0) gamma = asydec;
+      lo = xv - gamma * (o1 - low[i]);
+      up = xv + gamma * (upp[i] - o1);
+      const double lowmin = xv - 10.0 * (mx - mn), lowmax = xv - 0.01 * (mx - mn);
+      const double uppmin = xv + 0.01 * (mx - mn), uppmax = xv + 10.0 * (mx - mn);
+      lo = fmax(lo, lowmin);
+      lo = fmin(lo, lowmax);
+      up = fmin(up, uppmax);
+      up = fmax(up, uppmin);
+    }
+    low[i] = lo;
+    upp[i] = up;
+    double al = 0.9 * lo + 0.1 * xv;  // 1905-1908
+    if (al - mn < 0) al = mn;
+    double be = 0.9 * up + 0.1 * xv;
+    if (be - mx > 0) be = mx;
+    alpha[i] = al;
+    beta[i] = be;
+    const double ux = up - xv, xl = xv - lo, ul = up - lo;
+    const double df = dfdx[i], dg = dgdx[i];
+    p0[i] = ux * ux * (fmax(0.0, df) + 0.001 * fabs(df) + 0.5 * feps / ul);  // 1912
+    q0[i] = xl * xl * (fmax(0.0, -1.0 * df) + 0.001 * fabs(df) + 0.5 * feps / ul);
+    const double extra = 0.001 * fabs(dg) + 0.5 * feps / ul;  // 1918
+    const double pp = ux * ux * (fmax(0.0, dg) + extra), qq = xl * xl * (fmax(0.0, -1.0 * dg) + extra);
+    pij[i] = pp;
+    qij[i] = qq;
+    bs += pp / (up - xv) + qq / (xv - lo);  // 1927
+    xold2[i] = o1;  // 1867
+    xold1[i] = xv;  // 1868
+  }
+  double v[1] = {bs};
+  write_partials<1>(v, ws.partials);
+  (void)mm;
+  (void)nAct;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mma_gensub_fin(int nb, const double* __restrict__ partials, double* mm,
+                                                           double nAct) {
+  double v[1];
+  fold_partials<1>(partials, nb, v);
+  if (threadIdx.x != 0) return;
+  mm[MM_B] = -mm[MM_GX] + v[0];  // 1927
+  mm[MM_C] = 1000.0;             // 1815
+  mm[MM_A] = 0.0;
+  mm[MM_LAM] = mm[MM_C] / 2.0;   // 1933
+  mm[MM_MU] = 1.0;
+  mm[MM_TOL] = 1.0e-9 * sqrt(1.0 + nAct);  // 1935
+  mm[MM_EPSI] = 1.0;
+  mm[MM_ERR] = INFINITY;
+  mm[MM_LOOP] = 0.0;
+  mm[MM_DONE] = 0.0;
+  mm[MM_NEWTON] = 0.0;
+}
+
+__device__ __forceinline__ double mma_x(double lam, double p0, double q0, double pij, double qij, double lo,
+                                        double up, double al, double be, double* xp_out) {
+  const double pj = p0 + pij * lam, qj = q0 + qij * lam;
+  const double xp = (sqrt(pj) * lo + sqrt(qj) * up) / (sqrt(pj) + sqrt(qj));  // 1974
+  double xv = xp;
+  if (xv - al < 0) xv = al;
+  if (xv - be > 0) xv = be;
+  if (xp_out) *xp_out = xp;
+  return xv;
+}
+
+// Newton step part A: XYZofLAMBDA, DualGrad, DualHess, solve, DualLineSearch
+// (1944-1955).
+__global__ __launch_bounds__(kBlock) void k_mma_stepA(long long n, const uint8_t* __restrict__ emask,
+                                                      double* __restrict__ x, const double* __restrict__ low,
+                                                      const double* __restrict__ upp,
+                                                      const double* __restrict__ alpha,
+                                                      const double* __restrict__ beta, const double* __restrict__ p0,
+                                                      const double* __restrict__ q0, const double* __restrict__ pij,
+                                                      const double* __restrict__ qij, RedWs ws, double* mm) {
+  if (mm[MM_DONE] != 0.0) return;
+  double lam = mm[MM_LAM];
+  if (lam < 0.0) lam = 0.0;
+  double s[2] = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (!emask[i]) continue;
+    const double lo = low[i], up = upp[i], al = alpha[i], be = beta[i];
+    const double a0 = p0[i], b0 = q0[i], pp = pij[i], qq = qij[i];
+    double xp;
+    const double xv = mma_x(lam, a0, b0, pp, qq, lo, up, al, be, &xp);
+    x[i] = xv;
+    const double ux = up - xv, xl = xv - lo;
+    s[0] += pp / ux + qq / xl;  // 1985
+    const double pj = a0 + pp * lam, qj = b0 + qq * lam;
+    const double PQ = pp / (ux * ux) - qq / (xl * xl);  // 1993
+    double df2 = -1.0 / (2.0 * pj / (ux * ux * ux) + 2.0 * qj / (xl * xl * xl));
+    if (xp - al < 0) df2 = 0.0;
+    if (xp - be > 0) df2 = 0.0;
+    s[1] += PQ * df2 * PQ;  // 2000
+  }
+  write_partials<2>(s, ws.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mma_stepA_fin(int nb, const double* __restrict__ partials, double* mm) {
+  if (mm[MM_DONE] != 0.0) return;
+  double s[2];
+  fold_partials<2>(partials, nb, s);
+  if (threadIdx.x != 0) return;
+  double lam = mm[MM_LAM];
+  if (lam < 0.0) lam = 0.0;
+  const double c = mm[MM_C], a = mm[MM_A], epsi = mm[MM_EPSI];
+  double mu = mm[MM_MU];
+  // XYZofLAMBDA scalars (1967-1970)
+  double y = lam - c;
+  if (y < 0.0) y = 0.0;
+  const double z = fmax(0.0, 10.0 * (lam * a - 1.0));
+  // DualGrad (1984-1985) and 1947
+  double grad = -mm[MM_B] - a * z - y + s[0];
+  grad = -1.0 * grad - epsi / lam;
+  // DualHess (2000-2021)
+  double H = s[1];
+  const double lamai = lam * a;
+  if (lam > c) H = H - 1.0;
+  H = H - mu / lam;
+  if (lamai > 1.0) H = H - 10.0 * a * a;
+  double corr = 1e-4 * H / 1.0;
+  if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+  H = H + corr;
+  const double s1 = grad / H;                              // 1951
+  const double s2 = -mu + epsi / lam - s1 * mu / lam;      // 1953
+  double theta = 1.005;                                    // 2026-2031
+  if (theta < -1.01 * s1 / lam) theta = -1.01 * s1 / lam;
+  if (theta < -1.01 * s2 / mu) theta = -1.01 * s2 / mu;
+  theta = 1.0 / theta;
+  mm[MM_LAM] = lam + theta * s1;
+  mm[MM_MU] = mu + theta * s2;
+  mm[MM_Y] = y;
+  mm[MM_Z] = z;
+  mm[MM_GRAD] = grad;
+  mm[MM_HESS] = H;
+  mm[MM_S1] = s1;
+  mm[MM_S2] = s2;
+}
+
+// Newton step part B: XYZofLAMBDA + DualResidual and the loop control
+// (1941, 1957-1960).
+__global__ __launch_bounds__(kBlock) void k_mma_stepB(long long n, const uint8_t* __restrict__ emask,
+                                                      double* __restrict__ x, const double* __restrict__ low,
+                                                      const double* __restrict__ upp,
+                                                      const double* __restrict__ alpha,
+                                                      const double* __restrict__ beta, const double* __restrict__ p0,
+                                                      const double* __restrict__ q0, const double* __restrict__ pij,
+                                                      const double* __restrict__ qij, RedWs ws, double* mm) {
+  if (mm[MM_DONE] != 0.0) return;
+  double lam = mm[MM_LAM];
+  if (lam < 0.0) lam = 0.0;
+  double s[2] = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (!emask[i]) continue;
+    const double lo = low[i], up = upp[i];
+    const double xv = mma_x(lam, p0[i], q0[i], pij[i], qij[i], lo, up, alpha[i], beta[i], nullptr);
+    x[i] = xv;
+    s[0] += pij[i] / (up - xv);
+    s[1] += qij[i] / (xv - lo);
+  }
+  write_partials<2>(s, ws.partials);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mma_stepB_fin(int nb, const double* __restrict__ partials, double* mm) {
+  if (mm[MM_DONE] != 0.0) return;
+  double s[2];
+  fold_partials<2>(partials, nb, s);
+  if (threadIdx.x != 0) return;
+  double lam = mm[MM_LAM];
+  if (lam < 0.0) lam = 0.0;
+  const double c = mm[MM_C], a = mm[MM_A], epsi = mm[MM_EPSI], mu = mm[MM_MU];
+  double y = lam - c;
+  if (y < 0.0) y = 0.0;
+  const double z = fmax(0.0, 10.0 * (lam * a - 1.0));
+  mm[MM_LAM] = lam;
+  mm[MM_Y] = y;
+  mm[MM_Z] = z;
+  double r1 = -mm[MM_B] - a * z - y + mu;  // 2039-2041
+  r1 = r1 + s[0] + s[1];
+  const double r2 = mu * lam - epsi;
+  const double err = fmax(fabs(r1), fabs(r2));
+  mm[MM_ERR] = err;
+  double loop = mm[MM_LOOP] + 1.0;
+  mm[MM_NEWTON] = mm[MM_NEWTON] + 1.0;
+  if (!(err > 0.9 * epsi && loop < 100)) {  // 1941
+    const double e2 = epsi * 0.1;          // 1960
+    mm[MM_EPSI] = e2;
+    loop = 0.0;
+    if (!(e2 > mm[MM_TOL])) mm[MM_DONE] = 1.0;  // 1938
+  }
+  mm[MM_LOOP] = loop;
+}
+
+void launch_mma_gensub(hipStream_t s, long long n, const uint8_t* emask, int iter, const double* x,
+                       const double* xmin, const double* xmax, double* xold1, double* xold2, double* low,
+                       double* upp, const double* dfdx, const double* dgdx, double* alpha, double* beta,
+                       double* p0, double* q0, double* pij, double* qij, const RedWs& ws, double* mm, double nAct) {
+  hipLaunchKernelGGL(k_mma_gensub, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, iter, x, xmin, xmax, xold1,
+                     xold2, low, upp, dfdx, dgdx, alpha, beta, p0, q0, pij, qij, ws, mm, nAct);
+  hipLaunchKernelGGL(k_mma_gensub_fin, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, mm, nAct);
+}
+void launch_mma_step(hipStream_t s, long long n, const uint8_t* emask, double* x, const double* low,
+                     const double* upp, const double* alpha, const double* beta, const double* p0, const double* q0,
+                     const double* pij, const double* qij, const RedWs& ws, double* mm) {
+  hipLaunchKernelGGL(k_mma_stepA, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, x, low, upp, alpha, beta, p0,
+                     q0, pij, qij, ws, mm);
+  hipLaunchKernelGGL(k_mma_stepA_fin, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, mm);
+  hipLaunchKernelGGL(k_mma_stepB, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, x, low, upp, alpha, beta, p0,
+                     q0, pij, qij, ws, mm);
+  hipLaunchKernelGGL(k_mma_stepB_fin, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, mm);
+}
+
+}  // namespace t3x

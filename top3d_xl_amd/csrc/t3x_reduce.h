@@ -1,0 +1,112 @@
+// Deterministic device-wide reductions for wave64 (gfx950).
+//
+// Every reduction kernel runs a fixed grid; each block writes one partial in
+// slot order and a one-block finaliser kernel folds the partials in a fixed
+// order and applies the scalar logic (CG step lengths, convergence flags,
+// bisection decisions).  Results are bit-identical from run to run for a
+// given problem size, which the "same alone and in the mix" serving check
+// relies on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace t3x {
+
+constexpr int kBlock = 256;
+constexpr int kMaxRedBlocks = 2048;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Block-wide sum of NV values per thread (result valid in thread 0).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double sh[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) sh[i][w] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < kBlock / 64; ++k) s += sh[i][k];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Producer side: block sums of NV values to partials[i*gridDim.x + blockIdx.x].
+// No fence and no atomics: the consumer is the next kernel on the stream
+// (device-scope fences write back the per-XCD L2 and cost tens of
+// microseconds on a streaming kernel; a kernel boundary costs ~3 us).
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* partials) {
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) partials[(size_t)i * gridDim.x + blockIdx.x] = v[i];
+  }
+}
+
+__device__ __forceinline__ void write_partial_max(double v, double* partials) {
+  __shared__ double sh[kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = wave_max(v);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int k = 1; k < kBlock / 64; ++k) m = fmax(m, sh[k]);
+    partials[blockIdx.x] = m;
+  }
+}
+
+// Consumer side (one block of kBlock threads): fixed-order fold of nb
+// partials per value; totals valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void fold_partials(const double* partials, int nb, double (&out)[NV]) {
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[(size_t)i * nb + b];
+    out[i] = s;
+  }
+  block_sum<NV>(out);
+}
+
+__device__ __forceinline__ double fold_max(const double* partials, int nb) {
+  __shared__ double sh[kBlock / 64];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) m = fmax(m, partials[b]);
+  m = wave_max(m);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 1; k < kBlock / 64; ++k) m = fmax(m, sh[k]);
+  if (threadIdx.x == 0) m = fmax(m, sh[0]);
+  return m;
+}
+
+inline int red_blocks(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > kMaxRedBlocks) b = kMaxRedBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+}  // namespace t3x

@@ -1,0 +1,285 @@
+"""ctypes binding to libt3x.so and the Python mirror of the reference entry point.
+
+``TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin)`` keeps the
+signature of TOP3D_XL.m:4-19: 'GLOBAL' runs the compliance minimisation loop
+(TOP3D_XL_TO), 'LOCAL' the porous-infill loop (TOP3D_XL_PIO) and needs a 6th
+argument rHat.  Instead of writing files it returns the densities and the
+histories the reference writes to ./out/ (185-204, 402-421).
+
+All computation runs in the HIP library; there is no CPU path.  If the shared
+library is missing or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libt3x.so")
+
+T3X_OK = 0
+T3X_WARN_MAXIT = 100
+_STATUS = {1: "input", 2: "no loads", 3: "no fixations", 4: "HIP error", 5: "coarsest matrix not PD",
+           6: "no GPU device"}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class Params(C.Structure):
+    _fields_ = [("E0", C.c_double), ("nu", C.c_double), ("Emin", C.c_double), ("penal", C.c_double),
+                ("tol", C.c_double), ("maxIT", C.c_int), ("omega", C.c_double), ("coarsestControl", C.c_int),
+                ("precision", C.c_int), ("checkEvery", C.c_int), ("timing", C.c_int)]
+
+
+class Iter(C.Structure):
+    _fields_ = [("loop", C.c_int), ("pcgIts", C.c_int), ("filterIts", C.c_int), ("c", C.c_double),
+                ("vol", C.c_double), ("cons", C.c_double), ("sharp", C.c_double), ("change", C.c_double),
+                ("res", C.c_double), ("t", C.c_double * 6)]
+
+
+class Result(C.Structure):
+    _fields_ = [("xPhys", C.c_void_p), ("cHist", C.c_void_p), ("volHist", C.c_void_p),
+                ("consHist", C.c_void_p), ("sharpHist", C.c_void_p), ("lssIts", C.c_void_p),
+                ("tHist", C.c_void_p), ("xSnap", C.c_void_p), ("nSnap", C.c_int), ("resHist", C.c_void_p),
+                ("cSolid", C.c_double),
+                ("loops", C.c_int), ("warnings", C.c_int)]
+
+
+_lib = None
+
+
+def load_library():
+    """Loads libt3x.so (built by top3d_xl_amd/build.py); raises if missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run top3d_xl_amd/build.py (there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.t3x_default_params.argtypes = [C.POINTER(Params)]
+    L.t3x_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                             C.c_void_p, C.c_void_p, C.POINTER(Params), C.POINTER(vp)]
+    L.t3x_destroy.argtypes = [vp]
+    L.t3x_last_error.restype = C.c_char_p
+    L.t3x_last_error.argtypes = [vp]
+    L.t3x_info.argtypes = [vp, C.c_void_p]
+    L.t3x_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(Result)]
+    L.t3x_opt_begin.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double]
+    L.t3x_opt_step.argtypes = [vp, C.POINTER(Iter), C.POINTER(C.c_int)]
+    L.t3x_opt_result.argtypes = [vp, C.POINTER(Result)]
+    L.t3x_kernel_time.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+    L.t3x_kernel_time_reset.argtypes = [vp]
+    L.t3x_set_modulus.argtypes = [vp, _dp]
+    L.t3x_assemble.argtypes = [vp]
+    L.t3x_apply_K.argtypes = [vp, _dp, _dp]
+    L.t3x_vcycle.argtypes = [vp, _dp, _dp]
+    L.t3x_pcg.argtypes = [vp, _dp, _dp, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.t3x_compliance.argtypes = [vp, _dp, _dp]
+    L.t3x_filter_setup.argtypes = [vp, C.c_int, C.c_double]
+    L.t3x_filter.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
+    L.t3x_oc.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.t3x_mma.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp,
+                          C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+EXPORTED = ["t3x_default_params", "t3x_create", "t3x_destroy", "t3x_last_error", "t3x_info", "t3x_run",
+            "t3x_opt_begin", "t3x_opt_step", "t3x_opt_result", "t3x_kernel_time", "t3x_kernel_time_reset",
+            "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance",
+            "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma"]
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load_library().t3x_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class T3XError(RuntimeError):
+    pass
+
+
+class Problem:
+    """One voxel model on the GPU (state after CreateVoxelFEAmodel + FEA set-up)."""
+
+    def __init__(self, voxels, fix=None, load=None, **params):
+        L = load_library()
+        vox = np.asfortranarray(np.asarray(voxels).astype(np.uint8))
+        if vox.ndim != 3:
+            raise T3XError("Inappropriate Input Model!")
+        nely, nelx, nelz = vox.shape
+        flat = np.ascontiguousarray(vox.reshape(-1, order="F"))
+        self._keep = [flat]
+        P = default_params(**params)
+        h = C.c_void_p()
+        if fix is None:
+            rc = L.t3x_create(flat.ctypes.data, nely, nelx, nelz, -1, None, None, 0, None, None, C.byref(P),
+                              C.byref(h))
+        else:
+            fn = np.ascontiguousarray(fix[0], dtype=np.int32)
+            fs = np.ascontiguousarray(fix[1], dtype=np.uint8).reshape(-1)
+            ln = np.ascontiguousarray(load[0], dtype=np.int32)
+            lv = np.ascontiguousarray(load[1], dtype=np.float64).reshape(-1)
+            self._keep += [fn, fs, ln, lv]
+            rc = L.t3x_create(flat.ctypes.data, nely, nelx, nelz, len(fn), fn.ctypes.data, fs.ctypes.data, len(ln),
+                              ln.ctypes.data, lv.ctypes.data, C.byref(P), C.byref(h))
+        self.L = L
+        self.h = h
+        if rc != T3X_OK:
+            msg = L.t3x_last_error(h).decode() if h else _STATUS.get(rc, str(rc))
+            if h:
+                L.t3x_destroy(h)
+            self.h = None
+            raise T3XError(f"t3x_create failed ({_STATUS.get(rc, rc)}): {msg}")
+        info = np.zeros(1 + 8 * 16, dtype=np.int32)
+        L.t3x_info(h, info.ctypes.data)
+        self.num_levels = int(info[0])
+        self.levels = []
+        for l in range(self.num_levels):
+            o = info[1 + 8 * l: 9 + 8 * l]
+            self.levels.append(dict(resX=int(o[0]), resY=int(o[1]), resZ=int(o[2]), span=int(o[3]),
+                                    numElements=int(o[4]), numNodes=int(o[5])))
+        self.nE = self.levels[0]["numElements"]
+        self.nN = self.levels[0]["numNodes"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.t3x_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc not in (T3X_OK, T3X_WARN_MAXIT):
+            raise T3XError(f"{_STATUS.get(rc, rc)}: {self.L.t3x_last_error(self.h).decode()}")
+        return rc
+
+    # ------------------------------------------------------------- steps
+    def set_modulus(self, E):
+        self._check(self.L.t3x_set_modulus(self.h, np.ascontiguousarray(E, dtype=np.float64)))
+
+    def assemble(self):
+        self._check(self.L.t3x_assemble(self.h))
+
+    def apply_K(self, u):
+        y = np.zeros(3 * self.nN)
+        self._check(self.L.t3x_apply_K(self.h, np.ascontiguousarray(u, dtype=np.float64), y))
+        return y
+
+    def vcycle(self, r):
+        x = np.zeros(3 * self.nN)
+        self._check(self.L.t3x_vcycle(self.h, np.ascontiguousarray(r, dtype=np.float64), x))
+        return x
+
+    def pcg(self, b, x0=None, tol=1e-3, maxit=800):
+        x = np.zeros(3 * self.nN) if x0 is None else np.array(x0, dtype=np.float64)
+        its = C.c_int(0)
+        res = C.c_double(0.0)
+        self._check(self.L.t3x_pcg(self.h, np.ascontiguousarray(b, dtype=np.float64), x, tol, maxit,
+                                   C.byref(its), C.byref(res)))
+        return x, its.value, res.value
+
+    def compliance(self, U):
+        ce = np.zeros(self.nE)
+        self._check(self.L.t3x_compliance(self.h, np.ascontiguousarray(U, dtype=np.float64), ce))
+        return ce
+
+    def filter_setup(self, slot, r):
+        self._check(self.L.t3x_filter_setup(self.h, slot, float(r)))
+
+    def filter(self, slot, src):
+        dst = np.zeros(self.nE)
+        its = C.c_int(0)
+        self._check(self.L.t3x_filter(self.h, slot, np.ascontiguousarray(src, dtype=np.float64), dst,
+                                      C.byref(its)))
+        return dst, its.value
+
+    def oc(self, x, xPhys, dc, V0):
+        xx = np.array(x, dtype=np.float64)
+        ch = C.c_double(0.0)
+        nb = C.c_int(0)
+        self._check(self.L.t3x_oc(self.h, xx, np.ascontiguousarray(xPhys, dtype=np.float64),
+                                  np.ascontiguousarray(dc, dtype=np.float64), float(V0), C.byref(ch), C.byref(nb)))
+        return xx, ch.value, nb.value
+
+    def mma(self, iter_, xval, xmin, xmax, xold1, xold2, dfdx, gx, dgdx, low, upp):
+        arrs = [np.array(a, dtype=np.float64) for a in (xval, xmin, xmax, xold1, xold2, dfdx, dgdx, low, upp)]
+        xv, xmn, xmx, xo1, xo2, df, dg, lo, up = arrs
+        nn = C.c_int(0)
+        self._check(self.L.t3x_mma(self.h, int(iter_), xv, xmn, xmx, xo1, xo2, df, float(gx), dg, lo, up,
+                                   C.byref(nn)))
+        return dict(x=xv, xold1=xo1, xold2=xo2, low=lo, upp=up, newton=nn.value)
+
+    # ---------------------------------------------- iteration-level driver
+    def opt_begin(self, mode, V0, nLoop, rMin, rHat=0.0):
+        m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
+        self._check(self.L.t3x_opt_begin(self.h, m, float(V0), int(nLoop), float(rMin), float(rHat)))
+
+    def opt_step(self):
+        it = Iter()
+        fin = C.c_int(0)
+        self._check(self.L.t3x_opt_step(self.h, C.byref(it), C.byref(fin)))
+        rec = dict(loop=it.loop, pcgIts=it.pcgIts, filterIts=it.filterIts, c=it.c, vol=it.vol, cons=it.cons,
+                   sharp=it.sharp, change=it.change, res=it.res, t=list(it.t))
+        return rec, bool(fin.value)
+
+    def kernel_time(self, group):
+        """(launches, total ms) of a timed kernel group (needs timing=1)."""
+        n = C.c_longlong(0)
+        ms = C.c_double(0.0)
+        self._check(self.L.t3x_kernel_time(self.h, int(group), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def kernel_time_reset(self):
+        self._check(self.L.t3x_kernel_time_reset(self.h))
+
+    def run(self, mode, V0, nLoop, rMin, rHat=0.0, nSnap=0):
+        nE = self.nE
+        n = max(nLoop, 1)
+        xPhys = np.zeros(nE)
+        cH, vH, conH, sH = (np.zeros(n) for _ in range(4))
+        its = np.zeros(n, dtype=np.int32)
+        tH = np.zeros(n * 6)
+        snap = np.zeros(max(nSnap, 1) * nE)
+        resH = np.zeros(n)
+        R = Result(xPhys.ctypes.data, cH.ctypes.data, vH.ctypes.data, conH.ctypes.data, sH.ctypes.data,
+                   its.ctypes.data, tH.ctypes.data, snap.ctypes.data, int(nSnap), resH.ctypes.data, 0.0, 0, 0)
+        m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
+        self._check(self.L.t3x_run(self.h, m, float(V0), int(nLoop), float(rMin), float(rHat), C.byref(R)))
+        k = R.loops
+        return dict(xPhys=xPhys, cHist=cH[:k], volHist=vH[:k], consHist=conH[:k], sharpHist=sH[:k],
+                    lssIts=its[:k], resHist=resH[:k], tHist=tH[:6 * k].reshape(k, 6),
+                    xSnap=snap[:min(nSnap, k) * nE].reshape(-1, nE),
+                    cSolid=R.cSolid, loops=k, warnings=R.warnings)
+
+
+def TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin, **params):
+    """Drop-in for TOP3D_XL.m:4-19.  inputModel is a logical (nely,nelx,nelz)
+    array (the built-in cuboid BCs apply, 529-560) or a (voxels, fix, load)
+    tuple for explicit boundary conditions."""
+    if isinstance(inputModel, tuple):
+        vox, fix, load = inputModel
+    else:
+        vox, fix, load = inputModel, None, None
+    if consType == "GLOBAL":
+        rHat = 0.0
+        mode = 0
+    elif consType == "LOCAL":
+        if len(varargin) != 1:
+            raise T3XError("Wrong Input!")  # 16
+        rHat = float(varargin[0])
+        mode = 1
+    else:
+        raise T3XError("Wrong Input!")
+    P = Problem(vox, fix, load, **params)
+    try:
+        return P.run(mode, V0, nLoop, rMin, rHat)
+    finally:
+        P.close()
