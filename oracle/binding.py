@@ -61,6 +61,7 @@ def _load():
                           _dp, _dp, _dp, _dp, _dp, _ip, _dp, C.c_void_p, C.c_int,
                           C.POINTER(C.c_double), C.POINTER(C.c_int), _ip, _dp]
     L.orc_num_threads.restype = C.c_int
+    L.orc_set_order_variant.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -222,6 +223,10 @@ class Oracle:
                     lssIts=its[:k], filterIts=fits[:k], resHist=resH[:k], tHist=tH[:6 * k].reshape(k, 6),
                     xSnap=snap[:min(nSnap, k) * nE].reshape(-1, nE), cSolid=cS.value, loops=k,
                     warnings=self.L.orc_warnings(self.h))
+
+    def set_order_variant(self, k):
+        """k-th best coarsest-level ordering: same factor, different FP64 rounding."""
+        self.L.orc_set_order_variant(self.h, int(k))
 
     @staticmethod
     def num_threads():

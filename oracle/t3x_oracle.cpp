@@ -279,6 +279,7 @@ struct Model {
   std::vector<double> eleModulus;  // finest numElements
   // coarsest factor (banded Cholesky with our own node ordering)
   int cN = 0, cBW = 0;
+  int orderVariant = 0;           // 0: smallest-bandwidth axis order; k: the k-th best
   std::vector<int> cPerm;         // permuted dof -> compact dof
   std::vector<double> cBand;      // cN*(cBW+1): L(i, i-k) at i*(cBW+1)+k
   bool cholOk = false;
@@ -726,7 +727,7 @@ bool assemble(Model& M) {
     }
     const int dims[3] = {C.resY + 1, C.resX + 1, C.resZ + 1};
     const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-    int bestBW = -1; std::vector<int> bestOrd;
+    std::vector<std::pair<int, std::vector<int>>> cands;
     for (auto& pm : perms) {  // pm[0] fastest
       std::vector<i64> key(nn);
       for (int n = 0; n < nn; ++n) {
@@ -747,8 +748,15 @@ bool assemble(Model& M) {
         }
         bw = std::max(bw, hi - lo);
       }
-      if (bestBW < 0 || bw < bestBW) { bestBW = bw; bestOrd = ord; }
+      cands.push_back({bw, ord});
     }
+    std::stable_sort(cands.begin(), cands.end(),
+                     [](const std::pair<int, std::vector<int>>& a, const std::pair<int, std::vector<int>>& b) {
+                       return a.first < b.first;
+                     });
+    const size_t pick = std::min<size_t>((size_t)M.orderVariant, cands.size() - 1);
+    const int bestBW = cands[pick].first;
+    const std::vector<int>& bestOrd = cands[pick].second;
     std::vector<int> pos(nn);
     for (int k = 0; k < nn; ++k) pos[bestOrd[k]] = k;
     M.cN = 3 * nn;
@@ -1594,5 +1602,10 @@ int orc_run(void* h, int mode, double V0, int nLoop, double rMin, double rHat, d
 }
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+// Selects the k-th best coarsest-level node ordering (0 = smallest bandwidth).
+// Every ordering gives the same factor in exact arithmetic, as CHOLMOD's
+// fill-reducing permutation does; it only changes FP64 rounding.
+void orc_set_order_variant(void* h, int k) { ((Model*)h)->orderVariant = k; }
 
 }  // extern "C"

@@ -202,6 +202,19 @@ def test_galerkin_ks_diag_and_coarsest(case):
     np.testing.assert_allclose(o.coarse_solve(r), xr, rtol=0, atol=1e-9 * np.abs(xr).max())
 
 
+def test_coarse_solve_is_ordering_independent(case):
+    """The fill-reducing permutation of chol (1065) only changes rounding."""
+    o, g, rng = case["o"], case["g"], case["rng"]
+    r = rng.standard_normal(3 * g.nN[-1])
+    x0 = o.coarse_solve(r)
+    o.set_order_variant(2)
+    o.assemble()
+    x2 = o.coarse_solve(r)
+    o.set_order_variant(0)
+    o.assemble()
+    assert np.abs(x2 - x0).max() < 1e-9 * np.abs(x0).max()
+
+
 # ------------------------------------------------------- V-cycle (a6)
 def test_vcycle_is_the_dense_operator(case):
     o, g, E, rng = case["o"], case["g"], case["E"], case["rng"]

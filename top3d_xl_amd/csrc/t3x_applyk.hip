@@ -1,0 +1,200 @@
+// Tiled matrix-free K*u for gfx950 (Solving_KbyU_MatrixFree, TOP3D_XL.m:641-726).
+//
+// Node-centric and z-marching: a work-group owns a 16x16 tile of node columns
+// (y, x) and walks a chunk of node layers in z.  Per cell layer k it stages the
+// masked displacements of node layer k+1 (tile + 1-node halo, SoA, y fastest)
+// in LDS, double-buffered so the next layer's global loads overlap the FMAs.
+// Each thread keeps the 3x3 neighbourhoods of node layers k and k+1 in
+// registers and adds, for the 4 cells of layer k around its column, the rows
+// of E_e*Ke belonging to its node in layer k (which then is complete and is
+// written) and to its node in layer k+1 (carried to the next step).
+// Ke offsets are compile-time constants, so the coefficients are wave-uniform
+// scalar loads; every FMA reads one scalar and one vector register.
+//
+// Boundary conditions as in the reference (703-715, 1338-1437): fixed DOFs are
+// zeroed when staged (P_free u) and the fixed rows return u itself.
+#include <hip/hip_runtime.h>
+
+#include "t3x_kernels.h"
+#include "t3x_reduce.h"
+
+namespace t3x {
+
+constexpr int AK_TY = 16, AK_TX = 16, AK_T = AK_TY * AK_TX;  // 256 threads
+constexpr int AK_HY = AK_TY + 2, AK_HX = AK_TX + 2;          // halo tile
+constexpr int AK_ZC = 32;                                    // node layers per work-group
+
+template <typename T>
+struct AkTypes;
+template <>
+struct AkTypes<double> {
+  using acc = double;
+};
+template <>
+struct AkTypes<float> {
+  using acc = float;
+};
+
+// One cell's contribution: cell at column offset (cy, cx) in {0,1} relative to
+// (y-1, x-1); the thread's node sits at local (dy, dx) = (1-cy, 1-cx) in it.
+template <typename T, int CY, int CX>
+__device__ __forceinline__ void cell_pair(const T* __restrict__ Ke, const T (&n0)[27], const T (&n1)[27], T Ee,
+                                          T (&a0)[3], T (&a1)[3]) {
+  constexpr int dy = 1 - CY, dx = 1 - CX;
+  constexpr int jb = loc_index(dy, dx, 0);  // node in the bottom layer (k)
+  constexpr int jt = loc_index(dy, dx, 1);  // node in the top layer (k+1)
+  T s0[3] = {0, 0, 0}, s1[3] = {0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    // neighbourhood index of cell node j: row (y-1+CY+ky), col (x-1+CX+kx)
+    const int ky = CY + kLocDy[j], kx = CX + kLocDx[j];
+    const int nb = ky + 3 * kx;
+    const T* src = kLocDz[j] ? n1 : n0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v = src[nb * 3 + c];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        s0[r] += Ke[(3 * jb + r) * 24 + 3 * j + c] * v;
+        s1[r] += Ke[(3 * jt + r) * 24 + 3 * j + c] * v;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    a0[r] += s0[r] * Ee;
+    a1[r] += s1[r] * Ee;
+  }
+}
+
+template <typename T, typename TV>
+__global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restrict__ Ke, const TV* __restrict__ E,
+                                                        const uint8_t* __restrict__ fix, const TV* __restrict__ u,
+                                                        TV* __restrict__ y, double* __restrict__ partials,
+                                                        const int* fl, int reduce) {
+  if (fl && fl[FL_DONE]) return;
+  __shared__ T sU[2][3][AK_HX][AK_HY];
+  __shared__ T sE[2][AK_TX + 1][AK_TY + 1];
+  const int tid = threadIdx.x;
+  const int ty = tid % AK_TY, tx = tid / AK_TY;
+  const int y0 = blockIdx.x * AK_TY, x0 = blockIdx.y * AK_TX;
+  const int zc0 = blockIdx.z * AK_ZC;
+  const int zc1 = min(g.NZ, zc0 + AK_ZC);  // node layers written: [zc0, zc1)
+  const int yy = y0 + ty, xx = x0 + tx;
+  const bool own = yy < g.NY && xx < g.NX;
+  const long long NN = g.NN;
+  const long long plane = (long long)g.NY * g.NX;
+
+  // stage node layer z (masked u) into buffer b, and cell layer z's E into sE[b]
+  auto stage = [&](int z, int b) {
+    for (int t = tid; t < AK_HX * AK_HY; t += AK_T) {
+      const int hy = t % AK_HY, hx = t / AK_HY;
+      const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+      T v0 = 0, v1 = 0, v2 = 0;
+      if (z >= 0 && z < g.NZ && gy >= 0 && gx >= 0 && gy < g.NY && gx < g.NX) {
+        const long long n = gy + (long long)g.NY * gx + plane * z;
+        const uint8_t f = fix[n];
+        v0 = (f & 1) ? T(0) : T(u[n]);
+        v1 = (f & 2) ? T(0) : T(u[NN + n]);
+        v2 = (f & 4) ? T(0) : T(u[2 * NN + n]);
+      }
+      sU[b][0][hx][hy] = v0;
+      sU[b][1][hx][hy] = v1;
+      sU[b][2][hx][hy] = v2;
+    }
+    for (int t = tid; t < (AK_TX + 1) * (AK_TY + 1); t += AK_T) {
+      const int ey = t % (AK_TY + 1), ex = t / (AK_TY + 1);
+      const int gy = y0 - 1 + ey, gx = x0 - 1 + ex;
+      T e = 0;
+      if (z >= 0 && z < g.nz && gy >= 0 && gx >= 0 && gy < g.ny && gx < g.nx)
+        e = T(E[gy + (long long)g.NY * gx + plane * z]);
+      sE[b][ex][ey] = e;
+    }
+  };
+  auto read_nb = [&](int b, T (&nbv)[27]) {
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nbv[(ky + 3 * kx) * 3 + c] = sU[b][c][tx + kx][ty + ky];
+  };
+
+  T n0[27], n1[27];
+  T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+  double pq = 0.0;
+  // cell layers k = zc0-1 .. zc1-1; node layer k+1 staged in buffer (k+1)&1
+  const int kBeg = zc0 - 1;
+  stage(kBeg, kBeg & 1);  // node layer kBeg (may be -1: zeros) + cell layer kBeg E
+  __syncthreads();
+  read_nb(kBeg & 1, n0);
+  stage(kBeg + 1, (kBeg + 1) & 1);  // node layer kBeg+1, cell layer kBeg+1
+  for (int k = kBeg; k < zc1; ++k) {
+    __syncthreads();  // node layer k+1 and cell layer k+1 staged; cell layer k in sE[k&1]
+    const int bk = k & 1, bk1 = (k + 1) & 1;
+    read_nb(bk1, n1);
+    const T e00 = sE[bk][tx][ty], e10 = sE[bk][tx][ty + 1];
+    const T e01 = sE[bk][tx + 1][ty], e11 = sE[bk][tx + 1][ty + 1];
+    __syncthreads();  // buffer bk free for node layer k+2
+    if (k + 2 <= zc1) stage(k + 2, bk);
+    // cells (y-1+CY, x-1+CX) of layer k
+    if (e00 != T(0)) cell_pair<T, 0, 0>(Ke, n0, n1, e00, a0, a1);
+    if (e10 != T(0)) cell_pair<T, 1, 0>(Ke, n0, n1, e10, a0, a1);
+    if (e01 != T(0)) cell_pair<T, 0, 1>(Ke, n0, n1, e01, a0, a1);
+    if (e11 != T(0)) cell_pair<T, 1, 1>(Ke, n0, n1, e11, a0, a1);
+    if (k >= zc0 && own) {  // node layer k complete
+      const long long n = yy + (long long)g.NY * xx + plane * k;
+      const uint8_t f = fix[n];
+      const double u0 = u[n], u1 = u[NN + n], u2 = u[2 * NN + n];
+      const double r0 = (f & 1) ? u0 : double(a0[0]);
+      const double r1 = (f & 2) ? u1 : double(a0[1]);
+      const double r2 = (f & 4) ? u2 : double(a0[2]);
+      y[n] = TV(r0);
+      y[NN + n] = TV(r1);
+      y[2 * NN + n] = TV(r2);
+      pq += u0 * r0 + u1 * r1 + u2 * r2;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      a0[r] = a1[r];
+      a1[r] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 27; ++i) n0[i] = n1[i];
+  }
+  if (reduce) {
+    // one partial per work-group, flattened block index
+    __shared__ double shp[AK_T / 64];
+    double v = wave_sum(pq);
+    if ((tid & 63) == 0) shp[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < AK_T / 64; ++w) s += shp[w];
+      partials[blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z)] = s;
+    }
+  }
+}
+
+dim3 applyk_grid(const Geo& g) {
+  return dim3((g.NY + AK_TY - 1) / AK_TY, (g.NX + AK_TX - 1) / AK_TX, (g.NZ + AK_ZC - 1) / AK_ZC);
+}
+
+int applyk_tiled_blocks(const Geo& g) {
+  const dim3 d = applyk_grid(g);
+  return (int)(d.x * d.y * d.z);
+}
+
+void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                          const double* u, double* y, double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<double, double>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E, fix, u, y,
+                     partials, fl, partials ? 1 : 0);
+}
+
+void launch_apply_K_tiled_f32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                              const float* u, float* y, double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<float, float>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E, fix, u, y,
+                     partials, fl, partials ? 1 : 0);
+}
+
+}  // namespace t3x

@@ -27,6 +27,14 @@ struct RedWs {
 };
 
 // ---------------------------------------------------------------- FEM
+// tiled K*u (t3x_applyk.hip); partials: one per work-group (nullptr: none)
+int applyk_tiled_blocks(const Geo& g);
+void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                          const double* u, double* y, double* partials, const int* fl);
+void launch_apply_K_tiled_f32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                              const float* u, float* y, double* partials, const int* fl);
+void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                           const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                     const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_compliance(hipStream_t s, const Geo& g, const double* Ke, const uint8_t* emask, const double* u,

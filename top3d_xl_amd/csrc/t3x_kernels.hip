@@ -149,11 +149,17 @@ __global__ __launch_bounds__(kBlock) void k_apply_K(Geo g, const double* __restr
   }
 }
 
-void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                    const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
+void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                           const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
   const int nb = pcgStep ? red_blocks(g.NN) : grid_for(g.NN);
   hipLaunchKernelGGL(k_apply_K, dim3(nb), dim3(kBlock), 0, s, g, Ke, E, fix, u, y, ws, sc, fl, pcgStep ? 1 : 0);
   if (pcgStep) fin1(s, FIN_PQ, nb, ws, sc, (int*)fl, nullptr);
+}
+
+void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                    const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
+  launch_apply_K_tiled(s, g, Ke, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
+  if (pcgStep) fin1(s, FIN_PQ, applyk_tiled_blocks(g), ws, sc, (int*)fl, nullptr);
 }
 
 // =========================================================================
