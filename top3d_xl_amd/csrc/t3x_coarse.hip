@@ -40,34 +40,46 @@ struct BandView {
   int bw;
   __device__ __forceinline__ int W() const { return bw + 1; }
   __device__ __forceinline__ bool in(int i, int k) const { return i >= k && i - k <= bw && i < n && k >= 0; }
-  __device__ __forceinline__ double get(int i, int k) const { return in(i, k) ? B[(size_t)i * W() + (i - k)] : 0.0; }
+  // branch-free: the address is clamped into the band so loads can be batched
+  __device__ __forceinline__ double get(int i, int k) const {
+    const int d = i - k;
+    const bool ok = (d >= 0) & (d <= bw) & (i < n) & (k >= 0);
+    const size_t idx = (size_t)min(max(i, 0), n - 1) * W() + min(max(d, 0), bw);
+    const double v = B[idx];
+    return ok ? v : 0.0;
+  }
   __device__ __forceinline__ void set(int i, int k, double v) const {
     if (in(i, k)) B[(size_t)i * W() + (i - k)] = v;
   }
 };
 
 // ---- 64x64 tile helpers in LDS -----------------------------------------
-// Factor the lower triangle of sA (64x64 + pad) in place and write its
-// inverse into sI.  All threads of the block participate.
-__device__ void tile_potrf_inv(double (*sA)[TB + 1], double (*sI)[TB + 1], int* status) {
+// In-place Cholesky of the lower triangle of sA: two barriers per column, no
+// integer division (thread t owns row t/4 and columns t%4, t%4+4, ...).
+__device__ void tile_potrf(double (*sA)[TB + 1], int* status) {
+  const int row = threadIdx.x >> 2, c0 = threadIdx.x & 3;
   for (int c = 0; c < TB; ++c) {
+    const double d = sqrt(sA[c][c]);  // every thread reads the same pivot
+    if (row > c && c0 == 0) sA[row][c] = sA[row][c] / d;
+    __syncthreads();
     if (threadIdx.x == 0) {
-      const double d = sA[c][c];
       if (!(d > 0.0)) *status = 1;
-      sA[c][c] = sqrt(d);
+      sA[c][c] = d;
     }
-    __syncthreads();
-    const double d = sA[c][c];
-    for (int i = c + 1 + threadIdx.x; i < TB; i += CT) sA[i][c] /= d;
-    __syncthreads();
-    const int m = TB - 1 - c;  // trailing size
-    for (int t = threadIdx.x; t < m * m; t += CT) {
-      const int i = c + 1 + t / m, k = c + 1 + t % m;
-      if (k <= i) sA[i][k] -= sA[i][c] * sA[k][c];
+    if (row > c) {
+      const double lr = sA[row][c];
+#pragma unroll
+      for (int q = 0; q < TB / 4; ++q) {
+        const int k = c0 + 4 * q;
+        if (k > c && k <= row) sA[row][k] -= lr * sA[k][c];
+      }
     }
     __syncthreads();
   }
-  // inverse of the lower triangle, one column per thread
+}
+
+// Inverse of the lower-triangular sA into sI (one column per thread).
+__device__ void tile_trinv(const double (*sA)[TB + 1], double (*sI)[TB + 1]) {
   if (threadIdx.x < TB) {
     const int c = threadIdx.x;
     for (int i = 0; i < c; ++i) sI[i][c] = 0.0;
@@ -81,57 +93,76 @@ __device__ void tile_potrf_inv(double (*sA)[TB + 1], double (*sI)[TB + 1], int* 
   __syncthreads();
 }
 
+// acc[4][4] += A(4ti.., :) * B(:, 4tj..) for thread (ti, tj) = (t/16, t%16);
+// A given row-major sA[m][k], B given as sB[k][n] (transB = false) or
+// sB[n][k] (transB = true).
+template <bool transB>
+__device__ __forceinline__ void tile_gemm_acc(const double (*sA)[TB + 1], const double (*sB)[TB + 1],
+                                              double (&acc)[4][4]) {
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll 4
+  for (int k = 0; k < TB; ++k) {
+    double a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = sA[4 * ti + r][k];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = transB ? sB[4 * tj + q][k] : sB[k][4 * tj + q];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[r][q] += a[r] * b[q];
+  }
+}
+
 // ---- 1. banded Cholesky ---------------------------------------------------
-// Factor diagonal tile d0 (global row/col) in place and store its inverse.
-__device__ void diag_tile(BandView V, int d0, double* __restrict__ inv, double (*sA)[TB + 1],
-                          double (*sI)[TB + 1], int* status) {
+// Factor diagonal tile d0 (global row/col) in place.
+__device__ void diag_tile(BandView V, int d0, double (*sA)[TB + 1], int* status) {
   for (int t = threadIdx.x; t < TB * TB; t += CT) {
     const int i = t / TB, k = t % TB;
     sA[i][k] = V.get(d0 + i, d0 + k);
   }
   __syncthreads();
-  tile_potrf_inv(sA, sI, status);
+  tile_potrf(sA, status);
   for (int t = threadIdx.x; t < TB * TB; t += CT) {
     const int i = t / TB, k = t % TB;
     if (k <= i) V.set(d0 + i, d0 + k, sA[i][k]);
-    inv[t] = sI[i][k];
   }
 }
 
-__global__ __launch_bounds__(CT) void k_chol_diag(BandView V, int d0, double* __restrict__ tinv, int* status) {
+__global__ __launch_bounds__(CT) void k_chol_diag(BandView V, int d0, int* status) {
   __shared__ double sA[TB][TB + 1];
-  __shared__ double sI[TB][TB + 1];
-  diag_tile(V, d0, tinv + (size_t)(d0 / TB) * TB * TB, sA, sI, status);
+  diag_tile(V, d0, sA, status);
 }
 
-// L(i, J) = A(i, J) * L_JJ^{-T} for the row tiles below diagonal tile J.
-__global__ __launch_bounds__(CT) void k_chol_trsm(BandView V, int J, const double* __restrict__ tinv) {
-  __shared__ double sA[TB][TB + 1];
-  __shared__ double sI[TB][TB + 1];
+// L(i, J) = A(i, J) * L_JJ^{-T} for the row tiles below diagonal tile J, by
+// forward substitution (one row per thread, L_JJ in LDS): no tile inverse is
+// needed on the critical path.
+__global__ __launch_bounds__(CT) void k_chol_trsm(BandView V, int J) {
+  __shared__ double sL[TB][TB + 1];
+  __shared__ double sX[TB][TB + 1];
   const int j0 = J * TB, r0 = j0 + TB;
   const int i0 = r0 + blockIdx.x * TB;
-  const double* inv = tinv + (size_t)J * TB * TB;
   for (int t = threadIdx.x; t < TB * TB; t += CT) {
     const int i = t / TB, c = t % TB;
-    sA[i][c] = V.get(i0 + i, j0 + c);
-    sI[i][c] = inv[t];
+    sL[i][c] = V.get(j0 + i, j0 + c);
+    sX[i][c] = V.get(i0 + i, j0 + c);
   }
   __syncthreads();
-  double out[TB * TB / CT];
-  int q = 0;
-  for (int t = threadIdx.x; t < TB * TB; t += CT, ++q) {
-    const int i = t / TB, c = t % TB;
-    double s = 0.0;
-    for (int k = 0; k <= c; ++k) s += sA[i][k] * sI[c][k];
-    out[q] = s;
+  if (threadIdx.x < TB) {
+    const int i = threadIdx.x;
+    for (int c = 0; c < TB; ++c) {
+      double s = sX[i][c];
+      for (int k = 0; k < c; ++k) s -= sX[i][k] * sL[c][k];
+      sX[i][c] = s / sL[c][c];
+    }
   }
-  q = 0;
-  for (int t = threadIdx.x; t < TB * TB; t += CT, ++q) V.set(i0 + t / TB, j0 + t % TB, out[q]);
+  __syncthreads();
+  for (int t = threadIdx.x; t < TB * TB; t += CT) V.set(i0 + t / TB, j0 + t % TB, sX[t / TB][t % TB]);
 }
 
 // Trailing update of the band by panel J; tile 0 is the next diagonal tile,
 // which its work-group factors right after updating it.
-__global__ __launch_bounds__(CT) void k_chol_syrk(BandView V, int J, double* __restrict__ tinv, int* status) {
+__global__ __launch_bounds__(CT) void k_chol_syrk(BandView V, int J, int* status) {
   __shared__ double sA[TB][TB + 1];
   __shared__ double sB[TB][TB + 1];
   const int bw = V.bw, n = V.n;
@@ -148,19 +179,32 @@ __global__ __launch_bounds__(CT) void k_chol_syrk(BandView V, int J, double* __r
     sB[a][c] = V.get(k0 + a, j0 + c);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < TB * TB; t += CT) {
-    const int a = t / TB, b = t % TB;
-    const int i = i0 + a, k = k0 + b;
-    if (k > i || i - k > bw || i >= n) continue;
-    double s = 0.0;
-#pragma unroll 8
-    for (int c = 0; c < TB; ++c) s += sA[a][c] * sB[b][c];
-    V.B[(size_t)i * V.W() + (i - k)] -= s;
-  }
+  double acc[4][4] = {};
+  tile_gemm_acc<true>(sA, sB, acc);  // A(i, J) * A(k, J)^T
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + 4 * ti + r, k = k0 + 4 * tj + q;
+      if (k <= i && i - k <= bw && i < n) V.B[(size_t)i * V.W() + (i - k)] -= acc[r][q];
+    }
   if (t0 == 0) {
     __syncthreads();
-    diag_tile(V, r0, tinv + (size_t)(J + 1) * TB * TB, sA, sB, status);
+    diag_tile(V, r0, sA, status);
   }
+}
+
+// Inverses of all 64x64 diagonal tiles (after the factorization, in parallel).
+__global__ __launch_bounds__(CT) void k_tile_inverses(BandView V, double* __restrict__ tinv) {
+  __shared__ double sA[TB][TB + 1];
+  __shared__ double sI[TB][TB + 1];
+  const int d0 = blockIdx.x * TB;
+  for (int t = threadIdx.x; t < TB * TB; t += CT) sA[t / TB][t % TB] = V.get(d0 + t / TB, d0 + t % TB);
+  __syncthreads();
+  tile_trinv(sA, sI);
+  double* inv = tinv + (size_t)blockIdx.x * TB * TB;
+  for (int t = threadIdx.x; t < TB * TB; t += CT) inv[t] = sI[t / TB][t % TB];
 }
 
 // ---- 2. solve-block operators -------------------------------------------
@@ -198,6 +242,7 @@ __global__ __launch_bounds__(CT) void k_trsm_blocks(BandView V, int SB, const do
       sT[a][c] = r;
     }
     __syncthreads();
+    double acc[4][4] = {};
     for (int K = Ist; K < I; ++K) {
       if ((I * TB) - (K * TB + TB - 1) > V.bw) continue;  // L tile outside the band
       for (int t = threadIdx.x; t < TB * TB; t += CT) {
@@ -206,25 +251,33 @@ __global__ __launch_bounds__(CT) void k_trsm_blocks(BandView V, int SB, const do
         sX[a][c] = Out[(size_t)(K * TB + a) * SB + col0 + c];
       }
       __syncthreads();
-      for (int t = threadIdx.x; t < TB * TB; t += CT) {
-        const int a = t / TB, c = t % TB;
-        double s = 0.0;
-#pragma unroll 8
-        for (int k = 0; k < TB; ++k) s += sL[a][k] * sX[k][c];
-        sT[a][c] -= s;
-      }
+      tile_gemm_acc<false>(sL, sX, acc);
       __syncthreads();
     }
+    {
+      const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sT[4 * ti + r][4 * tj + q] -= acc[r][q];
+    }
+    __syncthreads();
     // X_I = inv(L_II) T
     const double* inv = tinv + (size_t)(base / TB + I) * TB * TB;
     for (int t = threadIdx.x; t < TB * TB; t += CT) sL[t / TB][t % TB] = inv[t];
     __syncthreads();
-    for (int t = threadIdx.x; t < TB * TB; t += CT) {
-      const int a = t / TB, c = t % TB;
-      double s = 0.0;
-      for (int k = 0; k <= a; ++k) s += sL[a][k] * sT[k][c];
-      Out[(size_t)(I * TB + a) * SB + col0 + c] = s;
-      if (ident) DT[(size_t)b * SB * SB + (size_t)(col0 + c) * SB + I * TB + a] = s;
+    {
+      double acc2[4][4] = {};
+      tile_gemm_acc<false>(sL, sT, acc2);  // inv(L_II) is lower triangular: the zeros add nothing
+      const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int a = 4 * ti + r, c = 4 * tj + q;
+          Out[(size_t)(I * TB + a) * SB + col0 + c] = acc2[r][q];
+          if (ident) DT[(size_t)b * SB * SB + (size_t)(col0 + c) * SB + I * TB + a] = acc2[r][q];
+        }
     }
     __syncthreads();
   }
@@ -242,9 +295,7 @@ __global__ __launch_bounds__(CT) void k_h_blocks(BandView V, int SB, int nblk, c
   const int I = rem / st, J = rem % st;
   if (b >= nblk - 1) return;
   const double* D = DT + (size_t)b * SB * SB;
-  double acc[TB * TB / CT];
-#pragma unroll
-  for (int q = 0; q < TB * TB / CT; ++q) acc[q] = 0.0;
+  double acc[4][4] = {};
   for (int K = I; K < st; ++K) {  // DT_b(i,k) = Dinv_b(k,i) is zero for k < i
     const int rowL0 = (b + 1) * SB + J * TB, colL0 = b * SB + K * TB;
     if (rowL0 - (colL0 + TB - 1) > V.bw) continue;
@@ -254,29 +305,31 @@ __global__ __launch_bounds__(CT) void k_h_blocks(BandView V, int SB, int nblk, c
       sB[a][c] = V.get(rowL0 + a, colL0 + c);  // row j, col k
     }
     __syncthreads();
-    int q = 0;
-    for (int t = threadIdx.x; t < TB * TB; t += CT, ++q) {
-      const int a = t / TB, c = t % TB;
-      double s = 0.0;
-#pragma unroll 8
-      for (int k = 0; k < TB; ++k) s += sA[a][k] * sB[c][k];
-      acc[q] += s;
-    }
+    tile_gemm_acc<true>(sA, sB, acc);
     __syncthreads();
   }
-  int q = 0;
-  for (int t = threadIdx.x; t < TB * TB; t += CT, ++q) {
-    const int a = t / TB, c = t % TB;
-    H[(size_t)b * SB * SB + (size_t)(I * TB + a) * SB + J * TB + c] = acc[q];
-  }
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      H[(size_t)b * SB * SB + (size_t)(I * TB + 4 * ti + r) * SB + J * TB + 4 * tj + q] = acc[r][q];
 }
 
 // ---- 3. the solve --------------------------------------------------------
 // One wave per row dot product: out[row] = base - sum_j M[row][j] v[j].
 __device__ __forceinline__ double wave_dot(const double* __restrict__ m, const double* __restrict__ v, int len) {
-  double s = 0.0;
   const int lane = threadIdx.x & 63;
-  for (int j = lane; j < len; j += 64) s += m[j] * v[j];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int j = lane;
+  for (; j + 192 < len; j += 256) {  // four independent loads in flight per lane
+    s0 += m[j] * v[j];
+    s1 += m[j + 64] * v[j + 64];
+    s2 += m[j + 128] * v[j + 128];
+    s3 += m[j + 192] * v[j + 192];
+  }
+  for (; j < len; j += 64) s0 += m[j] * v[j];
+  double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   return s;
@@ -343,18 +396,19 @@ void CoarseSolver::factor(hipStream_t s, double* band, double* tinv, double* Din
                           double* H, int* status) {
   BandView V{band, np, bw};
   const int nt = np / TB;
-  hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, 0, tinv, status);
+  hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, 0, status);
   for (int J = 0; J < nt; ++J) {
     const int j0 = J * TB, r0 = j0 + TB;
     const int rlast = std::min(np - 1, j0 + TB - 1 + bw);
     const int nrt = rlast >= r0 ? (rlast - r0) / TB + 1 : 0;
     if (nrt > 0) {
-      hipLaunchKernelGGL(k_chol_trsm, dim3(nrt), dim3(CT), 0, s, V, J, tinv);
-      hipLaunchKernelGGL(k_chol_syrk, dim3(nrt * (nrt + 1) / 2), dim3(CT), 0, s, V, J, tinv, status);
+      hipLaunchKernelGGL(k_chol_trsm, dim3(nrt), dim3(CT), 0, s, V, J);
+      hipLaunchKernelGGL(k_chol_syrk, dim3(nrt * (nrt + 1) / 2), dim3(CT), 0, s, V, J, status);
     } else if (J + 1 < nt) {
-      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, r0, tinv, status);
+      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, r0, status);
     }
   }
+  hipLaunchKernelGGL(k_tile_inverses, dim3(nt), dim3(CT), 0, s, V, tinv);
   const int st = SB / TB;
   hipLaunchKernelGGL(k_trsm_blocks, dim3(nblk * 2 * st), dim3(CT), 0, s, V, SB, tinv, Dinv, DT, Gm);
   if (nblk > 1) hipLaunchKernelGGL(k_h_blocks, dim3(nblk * st * st), dim3(CT), 0, s, V, SB, nblk, DT, H);

@@ -589,7 +589,7 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
     P.tend(TG_APPLYK, ek);
     launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                               // 616-622
     vcycle(P, P.r, P.z, sc, fl, true);                                                      // 628-629
-    launch_pcg_pupdate(s, P.p, P.z, n3, sc, fl);                                            // 630
+    launch_pcg_pupdate(s, y, P.p, P.z, n3, sc, fl);                                         // 616, 630
     if (it % ce == 0 || it == maxIT) {
       T3X_HIP(hipMemcpyAsync(P.hbuf, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       P.sync();
@@ -597,6 +597,7 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
       if (hf[FL_DONE]) { done = true; its = hf[FL_ITS]; break; }
     }
   }
+  launch_pcg_finish(s, y, P.p, n3, sc, fl);
   T3X_HIP(hipMemcpyAsync(P.hbuf, sc, SC_COUNT_PCG * sizeof(double), hipMemcpyDeviceToHost, s));
   P.sync();
   if (resOut) *resOut = P.hbuf[SC_RES];
@@ -662,7 +663,7 @@ int filter_run(t3x_problem& P, int slot, const double* src, double* dst) {
     launch_filter_apply(s, g0, kern, P.lv[0].emask, P.fp, P.fq, &P.ws, sc, fl);
     launch_pcg_update(s, P.fx, P.fr, P.fp, P.fq, NN, P.ws, sc, fl);
     launch_scale_diag(s, P.fz, dinv, P.fr, NN, &P.ws, sc, fl);
-    launch_pcg_pupdate(s, P.fp, P.fz, NN, sc, fl);
+    launch_pcg_pupdate(s, P.fx, P.fp, P.fz, NN, sc, fl);
     if (it % ce == 0 || it == maxIT) {
       T3X_HIP(hipMemcpyAsync(P.hbuf, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       P.sync();
@@ -670,6 +671,7 @@ int filter_run(t3x_problem& P, int slot, const double* src, double* dst) {
       if (hf[FL_DONE]) { its = hf[FL_ITS]; break; }
     }
   }
+  launch_pcg_finish(s, P.fx, P.fp, NN, sc, fl);
   launch_filter_n2e(s, g0, P.lv[0].emask, P.fx, dst);  // 894-901
   return its;
 }

@@ -75,7 +75,9 @@ void launch_norm2(hipStream_t s, const double* v, long long n, const RedWs& ws, 
 void launch_dot(hipStream_t s, const double* a, const double* b, long long n, const RedWs& ws, double* out);
 void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, const double* q, long long n,
                        const RedWs& ws, double* sc, int* fl);
-void launch_pcg_pupdate(hipStream_t s, double* p, const double* z, long long n, const double* sc, const int* fl);
+void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
+                        const int* fl);
+void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl);
 void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol);
 void launch_scale_diag(hipStream_t s, double* z, const double* dinv, const double* r, long long n,
                        const RedWs* ws, double* sc, const int* fl);

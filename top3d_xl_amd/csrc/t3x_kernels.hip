@@ -804,14 +804,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(double* __restrict__ y, d
                                                        const double* __restrict__ p, const double* __restrict__ q,
                                                        long long n, RedWs ws, double* sc, int* fl) {
   if (fl[FL_DONE]) return;
+  // y += lambda p (616) is deferred to k_pcg_pupdate, which reads p anyway
   const double lambda = sc[SC_LAMBDA];
   double rr = 0.0;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    y[i] = y[i] + lambda * p[i];                // 616
     const double ri = r[i] - lambda * q[i];     // 617
     r[i] = ri;
     rr += ri * ri;
   }
+  (void)y;
+  (void)p;
   double v[1] = {rr};
   write_partials<1>(v, ws.partials);
 }
@@ -821,15 +823,32 @@ void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, con
   fin1(s, FIN_RR, red_blocks(n), ws, sc, fl, nullptr);
 }
 
-__global__ void k_pcg_pupdate(double* __restrict__ p, const double* __restrict__ z, long long n,
-                              const double* sc, const int* fl) {
+// y += lambda p (616) for the iteration just finished, then p = z + beta p (630).
+__global__ void k_pcg_pupdate(double* __restrict__ y, double* __restrict__ p, const double* __restrict__ z,
+                              long long n, const double* sc, const int* fl) {
   if (fl[FL_DONE]) return;
-  const double beta = sc[SC_BETA];
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-    p[i] = z[i] + beta * p[i];  // 630
+  const double beta = sc[SC_BETA], lambda = sc[SC_LAMBDA];
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    const double pi = p[i];
+    y[i] = y[i] + lambda * pi;
+    p[i] = z[i] + beta * pi;
+  }
 }
-void launch_pcg_pupdate(hipStream_t s, double* p, const double* z, long long n, const double* sc, const int* fl) {
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, p, z, n, sc, fl);
+void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
+                        const int* fl) {
+  hipLaunchKernelGGL(k_pcg_pupdate, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, z, n, sc, fl);
+}
+// After the loop: the converging iteration skipped its p-update, so its
+// y += lambda p is applied here (no-op if the solve hit maxIT).
+__global__ void k_pcg_finish(double* __restrict__ y, const double* __restrict__ p, long long n, const double* sc,
+                             const int* fl) {
+  if (!fl[FL_DONE]) return;
+  const double lambda = sc[SC_LAMBDA];
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    y[i] = y[i] + lambda * p[i];
+}
+void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl) {
+  hipLaunchKernelGGL(k_pcg_finish, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, n, sc, fl);
 }
 
 __global__ void k_pcg_init(double* sc, int* fl, double tol) {
