@@ -7,6 +7,12 @@ assembly + coarsest Cholesky, MG-PCG solve, compliance, PDE filter, OC) of the
 rMin=sqrt(3).  Warm-up steps and the fully-solid reference solve run before
 the timed region; the model state is resident in HBM throughout.
 
+By default the precision form (FP32 K*p and V-cycle inside the FP64 outer CG)
+and the FP64 plain path run side by side on the same workload; `value` is the
+precision form's ms per design iteration, and the line carries the FP64
+path's figures, the MG-PCG speedup and the per-iteration compliance error of
+the precision form against the FP64 path.
+
 Prints ONE JSON line (see the keys below).  The CPU baseline is the FP64
 C++/OpenMP restatement in oracle/ (the reference itself is MATLAB and cannot
 run here), timed on a bounded sample on the host cores.
@@ -44,8 +50,10 @@ def gpu_run(args, precision):
     P = t3x.Problem(vox, timing=1, precision=precision, checkEvery=args.check_every)
     nN, nE = P.nN, P.nE
     P.opt_begin("GLOBAL", 0.12, args.warmup + args.steps, math.sqrt(3))
+    warm = []
     for _ in range(args.warmup):
-        P.opt_step()
+        rec, _ = P.opt_step()
+        warm.append(rec)
     P.kernel_time_reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -68,6 +76,7 @@ def gpu_run(args, precision):
     return dict(steps=steps, wall_s=t1 - t0, ms_per_step=1e3 * (t1 - t0) / steps, pcg_its=its,
                 pcg_dof_per_s=3 * nN * its / cg_s, cg_s=cg_s, nN=nN, nE=nE,
                 c_hist=[r["c"] for r in recs], its_hist=[r["pcgIts"] for r in recs],
+                c_all=[r["c"] for r in warm + recs], its_all=[r["pcgIts"] for r in warm + recs],
                 t_rows=[r["t"] for r in recs], applyK=(kn, kms), vcycle=(vn, vms), coarse=(cn, cms),
                 assemble=(an, ams))
 
@@ -95,7 +104,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", default="256x512x256", help="nely x nelx x nelz")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -106,19 +115,44 @@ def main():
                                    "multi-GPU design (x-slab decomposition) is documented, not run",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    prec = {"fp64": 0, "mixed": 1}[args.precision]
-    g = gpu_run(args, prec)
+    runs = {}
+    if args.precision in ("fp64", "both"):
+        print("[bench] fp64 plain path", file=sys.stderr, flush=True)
+        runs["fp64"] = gpu_run(args, 0)
+    if args.precision in ("mixed", "both"):
+        print("[bench] mixed-precision path", file=sys.stderr, flush=True)
+        runs["mixed"] = gpu_run(args, 1)
+    head = "mixed" if "mixed" in runs else "fp64"
+    g = runs[head]
     ny, nx, nz = parse_size(args.size)
+
+    def summary(r):
+        return {"ms_per_iteration": round(r["ms_per_step"], 3),
+                "mg_pcg_ms_per_design_iteration": round(1e3 * r["cg_s"] / r["steps"], 3),
+                "pcg_dof_per_s": round(r["pcg_dof_per_s"], 1), "pcg_iterations": r["pcg_its"],
+                "pcg_ms_per_iteration": round(1e3 * r["cg_s"] / max(r["pcg_its"], 1), 4),
+                "applyK_avg_ms": round(r["applyK"][1] / max(r["applyK"][0], 1), 4),
+                "vcycle_avg_ms": round(r["vcycle"][1] / max(r["vcycle"][0], 1), 4),
+                "coarse_solve_avg_ms": round(r["coarse"][1] / max(r["coarse"][0], 1), 4),
+                "assemble_avg_ms": round(r["assemble"][1] / max(r["assemble"][0], 1), 3),
+                "phase_s_mean": {k: round(sum(t[i] for t in r["t_rows"]) / r["steps"], 4)
+                                 for i, k in enumerate(["assemble", "cg", "opt", "filter", "lvf", "total"])},
+                "compliance_all_iterations": r["c_all"], "pcg_its_all_iterations": r["its_all"]}
+
     kn, kms = g["applyK"]
     k_avg_s = kms / max(kn, 1) / 1e3
     flops = 1152.0 * g["nE"]
-    bytes_ = 48.0 * g["nN"] + 8.0 * g["nE"]
+    # algorithmic bytes of one fine K*u: read u (FP64), read E, write y
+    ebytes = 4.0 if head == "mixed" else 8.0
+    ybytes = 12.0 if head == "mixed" else 24.0
+    bytes_ = (24.0 + ybytes) * g["nN"] + ebytes * g["nE"]
     tflops = flops / k_avg_s / 1e12
     gbs = bytes_ / k_avg_s / 1e9
-    peak = FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS
+    peak = FP64_PEAK_TFLOPS if head == "fp64" else FP32_PEAK_TFLOPS
     roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(tflops / peak, 4), "traffic": None,
-                "kernel": "k_apply_K (fine-level K*u)", "launches": kn, "avg_ms": round(1e3 * k_avg_s, 4),
+                "kernel": "k_apply_K_tiled (fine-level K*p inside PCG, %s)" % head, "launches": kn,
+                "avg_ms": round(1e3 * k_avg_s, 4), "algorithmic_flop": flops,
                 "algorithmic_bytes": bytes_, "achieved_GBs_algorithmic": round(gbs, 1)}
     out = {
         "metric": "ms per TO iteration + MG-PCG DOF/s at 256x512x256; solves/s and p99 latency",
@@ -131,21 +165,24 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == 0 else "f64+f32",
+        "dtype": "f64" if head == "fp64" else "f64+f32",
         "data": "synthetic (built-in cuboid model, no dataset)",
         "config": {"workload": f"GLOBAL topology optimisation, cuboid {ny}x{nx}x{nz} voxels (nely x nelx x nelz), "
-                               f"{3 * g['nN']} DOF, V0=0.12, rMin=sqrt(3), built-in BCs", "precision": args.precision},
+                               f"{3 * g['nN']} DOF, V0=0.12, rMin=sqrt(3), built-in BCs",
+                   "precision": head if head == "fp64" else
+                   "mixed: FP64 outer CG + FP64 coarsest Cholesky, FP32 K*p and V-cycle"},
         "pcg_dof_per_s": round(g["pcg_dof_per_s"], 1),
         "pcg_iterations": g["pcg_its"],
         "pcg_ms_per_iteration": round(1e3 * g["cg_s"] / max(g["pcg_its"], 1), 4),
-        "phase_s_mean": {k: round(sum(r[i] for r in g["t_rows"]) / g["steps"], 4)
-                         for i, k in enumerate(["assemble", "cg", "opt", "filter", "lvf", "total"])},
-        "vcycle_avg_ms": round(g["vcycle"][1] / max(g["vcycle"][0], 1), 4),
-        "coarse_solve_avg_ms": round(g["coarse"][1] / max(g["coarse"][0], 1), 4),
-        "assemble_avg_ms": round(g["assemble"][1] / max(g["assemble"][0], 1), 3),
-        "compliance_hist": g["c_hist"],
         "roofline": roofline,
     }
+    for k, r in runs.items():
+        out[k] = summary(r)
+    if "fp64" in runs and "mixed" in runs:
+        a, b = runs["fp64"], runs["mixed"]
+        out["mg_pcg_speedup_mixed_vs_fp64"] = round((a["cg_s"] / a["steps"]) / (b["cg_s"] / b["steps"]), 3)
+        out["iteration_speedup_mixed_vs_fp64"] = round(a["ms_per_step"] / b["ms_per_step"], 3)
+        out["compliance_rel_err_mixed_vs_fp64"] = [abs(x - y) / abs(y) for x, y in zip(b["c_all"], a["c_all"])]
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))

@@ -45,7 +45,11 @@ typedef struct t3x_params {
   int maxIT;             /* maxIT_              (31) 800 */
   double omega;          /* weightFactorJacobi_ (32) 0.6 */
   int coarsestControl;   /* coarsestResolutionControl_ (33) 50000 */
-  int precision;         /* 0 = FP64 plain path, 1 = mixed (FP32 V-cycle + FP32 K*u inside FP64 CG) */
+  int precision;         /* 0 = FP64 plain path;
+                            1 = mixed: FP64 CG vectors, dots and step lengths,
+                                FP32 K*p and FP32 V-cycle levels, FP64 coarsest
+                                Cholesky, FP64 reliable updates of the residual;
+                            2 = FP64 CG and K*p with the FP32 V-cycle only */
   int checkEvery;        /* PCG convergence flag read-back interval (>=1) */
   int timing;            /* 1 = time kernel groups with HIP events (t3x_kernel_time) */
 } t3x_params;
@@ -129,11 +133,17 @@ int t3x_kernel_time_reset(t3x_problem* h);
 int t3x_set_modulus(t3x_problem* h, const double* E);
 /* Solving_AssembleFEAstencil (919-1066) */
 int t3x_assemble(t3x_problem* h);
-/* Solving_KbyU_MatrixFree, finest level (641-726) */
+/* Solving_KbyU_MatrixFree, finest level (641-726); always FP64 */
 int t3x_apply_K(t3x_problem* h, const double* u, double* y);
-/* Solving_Vcycle ('Adapted', 747-781) */
+/* Solving_Vcycle ('Adapted', 747-781); with params.precision = 1 the FP32
+ * V-cycle (FP64 coarsest Cholesky) */
 int t3x_vcycle(t3x_problem* h, const double* r, double* x);
-/* Solving_PCG(@Solving_KbyU_MatrixFree, @Solving_Vcycle, ...) (600-639); x holds x0 on entry */
+/* Solving_PCG(@Solving_KbyU_MatrixFree, @Solving_Vcycle, ...) (600-639); x holds x0 on entry.
+ * With params.precision = 1: FP64 y, r, p, dot products and step lengths,
+ * FP32 K*p (FP64 input, FP32 arithmetic and output) and FP32 V-cycle; the
+ * recursive residual is replaced by the true FP64 residual b - K*y whenever
+ * it has dropped tenfold and before the solve may stop, so the stopping
+ * test always sees the true residual. */
 int t3x_pcg(t3x_problem* h, const double* b, double* x, double tol, int maxIT, int* its, double* resnorm);
 /* TopOpti_ComputeUnitCompliance (579-598) */
 int t3x_compliance(t3x_problem* h, const double* U, double* ce);

@@ -492,6 +492,20 @@ bool apply_bc(Model& M) {
     }
   }
   M.fixNode = fn; M.fixState = fs;
+  {  // 1307-1308: a node listed twice keeps its first load row
+    std::vector<std::pair<int, int>> tmp;
+    for (size_t i = 0; i < M.loadNode.size(); ++i) tmp.push_back({M.loadNode[i], (int)i});
+    std::sort(tmp.begin(), tmp.end());
+    std::vector<int> ln; std::vector<double> lv;
+    int last = -1;
+    for (auto& p : tmp) {
+      if (p.first == last) continue;
+      last = p.first;
+      ln.push_back(p.first);
+      for (int c = 0; c < 3; ++c) lv.push_back(M.loadVal[(size_t)p.second * 3 + c]);
+    }
+    M.loadNode = ln; M.loadVal = lv;
+  }
   if (M.loadNode.empty()) { M.err = "No Loads!"; return false; }        // 1309
   if (M.fixNode.empty()) { M.err = "No Fixations!"; return false; }     // 1310
   const Level& L = M.L[0];

@@ -24,17 +24,6 @@ constexpr int AK_TY = 16, AK_TX = 16, AK_T = AK_TY * AK_TX;  // 256 threads
 constexpr int AK_HY = AK_TY + 2, AK_HX = AK_TX + 2;          // halo tile
 constexpr int AK_ZC = 32;                                    // node layers per work-group
 
-template <typename T>
-struct AkTypes;
-template <>
-struct AkTypes<double> {
-  using acc = double;
-};
-template <>
-struct AkTypes<float> {
-  using acc = float;
-};
-
 // One cell's contribution: cell at column offset (cy, cx) in {0,1} relative to
 // (y-1, x-1); the thread's node sits at local (dy, dx) = (1-cy, 1-cx) in it.
 template <typename T, int CY, int CX>
@@ -67,12 +56,13 @@ __device__ __forceinline__ void cell_pair(const T* __restrict__ Ke, const T (&n0
   }
 }
 
-template <typename T, typename TV>
-__global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restrict__ Ke, const TV* __restrict__ E,
-                                                        const uint8_t* __restrict__ fix, const TV* __restrict__ u,
-                                                        TV* __restrict__ y, double* __restrict__ partials,
-                                                        const int* fl, int reduce) {
-  if (fl && fl[FL_DONE]) return;
+// T: arithmetic and LDS type; TE, TU, TY: types of E, u and y in HBM.
+template <typename T, typename TE, typename TU, typename TY>
+__global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restrict__ Ke, const TE* __restrict__ E,
+                                                        const uint8_t* __restrict__ fix, const TU* __restrict__ u,
+                                                        TY* __restrict__ y, double* __restrict__ partials,
+                                                        const int* fl, int reduce, int gate) {
+  if (fl && (fl[FL_DONE] || (gate >= 0 && !fl[gate]))) return;
   __shared__ T sU[2][3][AK_HX][AK_HY];
   __shared__ T sE[2][AK_TX + 1][AK_TY + 1];
   const int tid = threadIdx.x;
@@ -146,13 +136,13 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restri
       const long long n = yy + (long long)g.NY * xx + plane * k;
       const uint8_t f = fix[n];
       const double u0 = u[n], u1 = u[NN + n], u2 = u[2 * NN + n];
-      const double r0 = (f & 1) ? u0 : double(a0[0]);
-      const double r1 = (f & 2) ? u1 : double(a0[1]);
-      const double r2 = (f & 4) ? u2 : double(a0[2]);
-      y[n] = TV(r0);
-      y[NN + n] = TV(r1);
-      y[2 * NN + n] = TV(r2);
-      pq += u0 * r0 + u1 * r1 + u2 * r2;
+      const TY r0 = (f & 1) ? TY(u0) : TY(a0[0]);
+      const TY r1 = (f & 2) ? TY(u1) : TY(a0[1]);
+      const TY r2 = (f & 4) ? TY(u2) : TY(a0[2]);
+      y[n] = r0;
+      y[NN + n] = r1;
+      y[2 * NN + n] = r2;
+      pq += u0 * double(r0) + u1 * double(r1) + u2 * double(r2);
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -186,15 +176,15 @@ int applyk_tiled_blocks(const Geo& g) {
 }
 
 void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                          const double* u, double* y, double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_tiled<double, double>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E, fix, u, y,
-                     partials, fl, partials ? 1 : 0);
+                          const double* u, double* y, double* partials, const int* fl, int gate) {
+  hipLaunchKernelGGL((k_apply_K_tiled<double, double, double, double>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E,
+                     fix, u, y, partials, fl, partials ? 1 : 0, gate);
 }
 
-void launch_apply_K_tiled_f32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                              const float* u, float* y, double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_tiled<float, float>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E, fix, u, y,
-                     partials, fl, partials ? 1 : 0);
+void launch_apply_K_tiled_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                                const double* u, float* y, double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<float, float, double, float>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E,
+                     fix, u, y, partials, fl, partials ? 1 : 0, -1);
 }
 
 }  // namespace t3x

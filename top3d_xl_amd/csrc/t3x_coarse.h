@@ -20,8 +20,12 @@ struct CoarseSolver {
   void pad_identity(hipStream_t s, double* band);
   void factor(hipStream_t s, double* band, double* tinv, double* Dinv, double* DT, double* Gm, double* H,
               int* status);
+  // FP64 solve; the right-hand side is gathered from / the solution scattered
+  // to dense level vectors of either precision
   void solve(hipStream_t s, const long long* map, const double* rdense, double* xdense, const double* Dinv,
-             const double* DT, const double* Gm, const double* H, double* va, double* vb, const int* fl);
+             const double* DT, const double* Gm, const double* H, double* va, double* vb, const int* fl) const;
+  void solve(hipStream_t s, const long long* map, const float* rdense, float* xdense, const double* Dinv,
+             const double* DT, const double* Gm, const double* H, double* va, double* vb, const int* fl) const;
 };
 
 }  // namespace t3x

@@ -336,8 +336,9 @@ __device__ __forceinline__ double wave_dot(const double* __restrict__ m, const d
 }
 
 // forward phase 0: gather the permuted right-hand side and va_b = Dinv_b v_b
+template <typename TR>
 __global__ __launch_bounds__(CT) void k_solve_f0(int n, int np, int SB, const long long* __restrict__ map,
-                                                 const double* __restrict__ rdense, const double* __restrict__ Dinv,
+                                                 const TR* __restrict__ rdense, const double* __restrict__ Dinv,
                                                  double* __restrict__ va, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int row = blockIdx.x * (CT / 64) + (threadIdx.x >> 6);
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(CT) void k_solve_f0(int n, int np, int SB, const lo
   double s = 0.0;
   for (int j = lane; j <= i; j += 64) {
     const int g = b * SB + j;
-    s += m[j] * (g < n ? rdense[map[g]] : 0.0);
+    s += m[j] * (g < n ? (double)rdense[map[g]] : 0.0);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -373,11 +374,12 @@ __global__ __launch_bounds__(CT) void k_solve_b0(int np, int SB, const double* _
   const double s = wave_dot(DT + (size_t)b * SB * SB + (size_t)i * SB + i, va + (size_t)b * SB + i, SB - i);
   if ((threadIdx.x & 63) == 0) vb[row] = s;
 }
+template <typename TX>
 __global__ void k_solve_scatter(int n, const long long* __restrict__ map, const double* __restrict__ vb,
-                                double* __restrict__ xdense, const int* fl) {
+                                TX* __restrict__ xdense, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int i = blockIdx.x * CT + threadIdx.x;
-  if (i < n) xdense[map[i]] = vb[i];
+  if (i < n) xdense[map[i]] = (TX)vb[i];
 }
 
 // ---- host side -----------------------------------------------------------
@@ -414,16 +416,31 @@ void CoarseSolver::factor(hipStream_t s, double* band, double* tinv, double* Din
   if (nblk > 1) hipLaunchKernelGGL(k_h_blocks, dim3(nblk * st * st), dim3(CT), 0, s, V, SB, nblk, DT, H);
 }
 
-void CoarseSolver::solve(hipStream_t s, const long long* map, const double* rdense, double* xdense, const double* Dinv,
-                         const double* DT, const double* Gm, const double* H, double* va, double* vb, const int* fl) {
+template <typename TV>
+static void solve_impl(const CoarseSolver& C, hipStream_t s, const long long* map, const TV* rdense, TV* xdense,
+                       const double* Dinv, const double* DT, const double* Gm, const double* H, double* va,
+                       double* vb, const int* fl) {
   const int wpb = CT / 64;
-  hipLaunchKernelGGL(k_solve_f0, dim3((np + wpb - 1) / wpb), dim3(CT), 0, s, n, np, SB, map, rdense, Dinv, va, fl);
+  const int n = C.n, np = C.np, SB = C.SB, nblk = C.nblk;
+  hipLaunchKernelGGL(k_solve_f0<TV>, dim3((np + wpb - 1) / wpb), dim3(CT), 0, s, n, np, SB, map, rdense, Dinv, va,
+                     fl);
   for (int b = 1; b < nblk; ++b)
     hipLaunchKernelGGL(k_solve_step, dim3((SB + wpb - 1) / wpb), dim3(CT), 0, s, SB, b, -1, Gm, va, fl);
   hipLaunchKernelGGL(k_solve_b0, dim3((np + wpb - 1) / wpb), dim3(CT), 0, s, np, SB, DT, va, vb, fl);
   for (int b = nblk - 2; b >= 0; --b)
     hipLaunchKernelGGL(k_solve_step, dim3((SB + wpb - 1) / wpb), dim3(CT), 0, s, SB, b, +1, H, vb, fl);
-  hipLaunchKernelGGL(k_solve_scatter, dim3((n + CT - 1) / CT), dim3(CT), 0, s, n, map, vb, xdense, fl);
+  hipLaunchKernelGGL(k_solve_scatter<TV>, dim3((n + CT - 1) / CT), dim3(CT), 0, s, n, map, vb, xdense, fl);
+}
+
+void CoarseSolver::solve(hipStream_t s, const long long* map, const double* rdense, double* xdense, const double* Dinv,
+                         const double* DT, const double* Gm, const double* H, double* va, double* vb,
+                         const int* fl) const {
+  solve_impl(*this, s, map, rdense, xdense, Dinv, DT, Gm, H, va, vb, fl);
+}
+void CoarseSolver::solve(hipStream_t s, const long long* map, const float* rdense, float* xdense, const double* Dinv,
+                         const double* DT, const double* Gm, const double* H, double* va, double* vb,
+                         const int* fl) const {
+  solve_impl(*this, s, map, rdense, xdense, Dinv, DT, Gm, H, va, vb, fl);
 }
 
 // identity rows beyond the true order (after band assembly)

@@ -164,6 +164,10 @@ struct t3x_problem {
   double* bvec = nullptr;
   double *F = nullptr, *U = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp3 = nullptr;
   double *E = nullptr, *x = nullptr, *xPhys = nullptr, *ce = nullptr, *dc = nullptr;
+  // mixed precision (precision == 1): FP32 Ke, E, K*p output and V-cycle output
+  float *Ke32 = nullptr, *E32 = nullptr, *z32 = nullptr, *q32 = nullptr;
+  void* rtmp = nullptr;  // intermediate of the separable restriction (largest level pair)
+  int lastReliable = 0;  // reliable updates in the last mixed solve
   double *ca = nullptr, *cb = nullptr, *cc = nullptr, *cd = nullptr;  // cell scratch
   // filters (node scalars)
   double *fb = nullptr, *fx = nullptr, *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr;
@@ -189,7 +193,7 @@ struct t3x_problem {
   // reductions / scalars
   RedWs ws{nullptr, nullptr};
   double* sc = nullptr;   // [0..15] elastic PCG, [16..31] filter PCG, [32..] misc
-  int* fl = nullptr;      // [0..3] elastic, [4..7] filter
+  int* fl = nullptr;      // [0..7] elastic, [8..15] filter
   double* oc = nullptr;   // 8
   double* mm = nullptr;   // MM_COUNT
   double* hbuf = nullptr; // pinned read-back
@@ -252,6 +256,7 @@ struct t3x_problem {
   double* xTilde() { return ca; }   // LOCAL-mode state between iterations
   double* volMax() { return cb; }
   long long NN() const { return lv[0].NN; }
+  bool mixed() const { return P.precision != 0; }  // FP32 V-cycle
   void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
 };
 
@@ -364,16 +369,20 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   std::vector<double> hF(3 * NN, 0.0);
   std::vector<uint8_t> loaded(NN, 0);
   int nL = 0, nF = 0;
+  // a node listed twice keeps its first entry (unique(), 499-518)
   for (size_t i = 0; i < loadD.size(); ++i) {
     const long long d = loadD[i];
-    if (d < 0 || d >= NN || !F.hNmask[d]) continue;
+    if (d < 0 || d >= NN || !F.hNmask[d] || loaded[d]) continue;
     loaded[d] = 1;
     for (int c = 0; c < 3; ++c) hF[c * NN + d] = lv[3 * i + c];
   }
   for (long long d = 0; d < NN; ++d) nL += loaded[d];
+  std::vector<uint8_t> fixSeen(NN, 0);
   for (size_t i = 0; i < fixD.size(); ++i) {
     const long long d = fixD[i];
-    if (d < 0 || d >= NN || !F.hNmask[d] || loaded[d]) continue;  // loaded nodes are released (1303)
+    if (d < 0 || d >= NN || !F.hNmask[d] || fixSeen[d]) continue;
+    fixSeen[d] = 1;
+    if (loaded[d]) continue;  // loaded nodes are released (1303)
     uint8_t m = 0;
     for (int c = 0; c < 3; ++c) m |= (fs[3 * i + c] ? 1 : 0) << c;
     hfix[d] = m;
@@ -396,6 +405,18 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       V.x = P.alloc<double>(3 * V.NN);
       V.Ks = P.alloc<double>((size_t)kTri * V.NN);
     }
+    if (P.P.precision != 0) {
+      if (l < P.L - 1) V.diagf = P.alloc<float>(3 * V.NN);
+      if (l >= 1) {
+        V.rf = P.alloc<float>(3 * V.NN);
+        V.xf = P.alloc<float>(3 * V.NN);
+      }
+    }
+  }
+  {
+    long long tl = 0;
+    for (int l = 0; l + 1 < P.L; ++l) tl = std::max(tl, restrict_tmp_len(geo(P.lv[l]), geo(P.lv[l + 1])));
+    P.rtmp = P.alloc<double>(tl);
   }
   P.fix = P.alloc<uint8_t>(NN);
   T3X_HIP(hipMemcpy(P.fix, hfix.data(), NN, hipMemcpyHostToDevice));
@@ -408,6 +429,14 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   const std::vector<double> Ke = gauss_ke(P.P.nu, 1.0);
   P.dKe = P.alloc<double>(576);
   T3X_HIP(hipMemcpy(P.dKe, Ke.data(), 576 * sizeof(double), hipMemcpyHostToDevice));
+  if (P.P.precision != 0) {
+    std::vector<float> Kf(Ke.begin(), Ke.end());
+    P.Ke32 = P.alloc<float>(576);
+    T3X_HIP(hipMemcpy(P.Ke32, Kf.data(), 576 * sizeof(float), hipMemcpyHostToDevice));
+    P.E32 = P.alloc<float>(NN);
+    P.z32 = P.alloc<float>(3 * NN);
+    P.q32 = P.alloc<float>(3 * NN);
+  }
   const std::vector<double> T2 = l2_templates(Ke, P.lv[1].span);
   P.dT2 = P.alloc<double>(T2.size());
   T3X_HIP(hipMemcpy(P.dT2, T2.data(), T2.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -528,6 +557,10 @@ int assemble(t3x_problem& P) {
     if (l < P.L - 1) launch_diag_coarse(s, gc, C.Ks, C.emask, C.nmask, C.diag);
   }
   launch_diag_fine(s, g0, P.dKe, P.E, P.fix, P.lv[0].nmask, P.lv[0].diag);
+  if (P.P.precision != 0) {
+    for (int l = 0; l < P.L - 1; ++l) launch_convert(s, P.lv[l].diagf, P.lv[l].diag, 3 * P.lv[l].NN);
+    launch_convert(s, P.E32, P.E, P.NN());
+  }
   const Level& C = P.lv[P.L - 1];
   launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, P.band, (long long)P.cs.band_len());
   P.cs.pad_identity(s, P.band);
@@ -543,55 +576,107 @@ int assemble(t3x_problem& P) {
 }
 
 // ------------------------------------------------------------- V-cycle
-// Solving_Vcycle, 'Adapted' (747-781).  red: 0 none, 1/2 fused z.r (initial / in-loop).
+// Solving_Vcycle, 'Adapted' (747-781); fuse: z.r reduction into the last
+// prolongation (612 / 629).
 void vcycle(t3x_problem& P, const double* r, double* z, double* sc, const int* fl, bool fuse) {
   hipStream_t s = P.stream;
   const int L = P.L;
+  double* tmp = (double*)P.rtmp;
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
-  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, P.lv[1].nmask, P.lv[1].r, fl);
+  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, tmp, P.lv[1].nmask, P.lv[1].r, fl);
   for (int l = 1; l < L - 1; ++l)
-    launch_restrict(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.lv[l].r, P.lv[l + 1].nmask,
-                    P.lv[l + 1].r, fl);
+    launch_restrict(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, (const double*)P.lv[l].r, tmp,
+                    P.lv[l + 1].nmask, P.lv[l + 1].r, fl);
   Level& C = P.lv[L - 1];
   P.tbegin(TG_COARSE, ec);
   P.cs.solve(s, P.dCMap, C.r, C.x, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);
   P.tend(TG_COARSE, ec);
   for (int l = L - 2; l >= 1; --l)
-    launch_prolong_jacobi(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.P.omega, P.lv[l + 1].x,
-                          P.lv[l].r, P.lv[l].diag, P.lv[l].nmask, P.lv[l].x, nullptr, nullptr, fl);
-  launch_prolong_jacobi(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, P.P.omega, P.lv[1].x, r, P.lv[0].diag,
-                        P.lv[0].nmask, z, fuse ? &P.ws : nullptr, sc, fl);
+    launch_prolong_jacobi(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.P.omega,
+                          (const double*)P.lv[l + 1].x, (const double*)P.lv[l].r, (const double*)P.lv[l].diag,
+                          P.lv[l].nmask, P.lv[l].x, nullptr, nullptr, fl);
+  launch_prolong_jacobi(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, P.P.omega, (const double*)P.lv[1].x, r,
+                        (const double*)P.lv[0].diag, P.lv[0].nmask, z, fuse ? &P.ws : nullptr, sc, fl);
+  P.tend(TG_VCYCLE, ev);
+}
+
+// The same operator with FP32 level vectors and smoother output; the fine
+// residual stays FP64 and the coarsest solve is the FP64 band Cholesky.
+void vcycle_mixed(t3x_problem& P, const double* r, float* z, double* sc, const int* fl, bool fuse) {
+  hipStream_t s = P.stream;
+  const int L = P.L;
+  float* tmp = (float*)P.rtmp;
+  hipEvent_t ev[2], ec[2];
+  P.tbegin(TG_VCYCLE, ev);
+  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
+  for (int l = 1; l < L - 1; ++l)
+    launch_restrict(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, (const float*)P.lv[l].rf, tmp,
+                    P.lv[l + 1].nmask, P.lv[l + 1].rf, fl);
+  Level& C = P.lv[L - 1];
+  P.tbegin(TG_COARSE, ec);
+  P.cs.solve(s, P.dCMap, C.rf, C.xf, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);
+  P.tend(TG_COARSE, ec);
+  for (int l = L - 2; l >= 1; --l)
+    launch_prolong_jacobi(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.P.omega,
+                          (const float*)P.lv[l + 1].xf, (const float*)P.lv[l].rf, (const float*)P.lv[l].diagf,
+                          P.lv[l].nmask, P.lv[l].xf, nullptr, nullptr, fl);
+  launch_prolong_jacobi(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, P.P.omega, (const float*)P.lv[1].xf, r,
+                        (const float*)P.lv[0].diagf, P.lv[0].nmask, z, fuse ? &P.ws : nullptr, sc, fl);
   P.tend(TG_VCYCLE, ev);
 }
 
 // ------------------------------------------------------------- PCG
 // Solving_PCG (600-639) with device-side scalars and convergence flag.
+// Reliable-update factor of the mixed form: a true FP64 residual replaces the
+// recursive one whenever the latter has dropped by this factor, and before
+// the solve may stop.
+constexpr double kReliableDelta = 0.1;
+
 int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxIT, double* resOut, bool* warned) {
   hipStream_t s = P.stream;
   const long long n3 = 3 * P.NN();
   double* sc = P.sc;
   int* fl = P.fl;
   const Geo g0 = geo(P.lv[0]);
-  launch_pcg_init(s, sc, fl, tol);
+  const int prec = P.P.precision;
+  launch_pcg_init(s, sc, fl, tol, prec == 1 ? kReliableDelta : 0.0);
   launch_norm2(s, b, n3, P.ws, &sc[SC_NORMB]);                                              // 607
-  launch_apply_K(s, g0, P.dKe, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);               // 609
+  launch_apply_K(s, g0, P.dKe, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);               // 609 (FP64)
   launch_sub(s, P.r, b, P.q, n3);
-  vcycle(P, P.r, P.z, sc, nullptr, true);                                                   // 610, 612
-  launch_copy(s, P.p, P.z, n3);                                                             // 611
+  if (prec) {
+    vcycle_mixed(P, P.r, P.z32, sc, nullptr, true);                                         // 610, 612
+    launch_convert(s, P.p, P.z32, n3);                                                      // 611
+  } else {
+    vcycle(P, P.r, P.z, sc, nullptr, true);
+    launch_copy(s, P.p, P.z, n3);
+  }
   const int ce = std::max(1, P.P.checkEvery);
   int its = maxIT;
   bool done = false;
   for (int it = 1; it <= maxIT; ++it) {
     hipEvent_t ek[2];
     P.tbegin(TG_APPLYK, ek);
-    launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                 // 614-615
+    if (prec == 1)
+      launch_apply_K_mixed(s, g0, P.Ke32, P.E32, P.fix, P.p, P.q32, P.ws, sc, fl, true);    // 614-615
+    else
+      launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);
     P.tend(TG_APPLYK, ek);
-    launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                               // 616-622
-    vcycle(P, P.r, P.z, sc, fl, true);                                                      // 628-629
-    launch_pcg_pupdate(s, y, P.p, P.z, n3, sc, fl);                                         // 616, 630
+    if (prec == 1) {
+      launch_pcg_update(s, P.r, P.q32, n3, P.ws, sc, fl);                                   // 617-622
+      launch_reliable_update(s, g0, P.dKe, P.E, P.fix, b, y, P.p, P.q, P.r, n3, P.ws, sc, fl);
+    } else {
+      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);
+    }
+    if (prec) {
+      vcycle_mixed(P, P.r, P.z32, sc, fl, true);                                            // 628-629
+      launch_pcg_pupdate(s, y, P.p, P.z32, n3, sc, fl);                                     // 616, 630
+    } else {
+      vcycle(P, P.r, P.z, sc, fl, true);
+      launch_pcg_pupdate(s, y, P.p, P.z, n3, sc, fl);
+    }
     if (it % ce == 0 || it == maxIT) {
-      T3X_HIP(hipMemcpyAsync(P.hbuf, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+      T3X_HIP(hipMemcpyAsync(P.hbuf, fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
       P.sync();
       const int* hf = (const int*)P.hbuf;
       if (hf[FL_DONE]) { done = true; its = hf[FL_ITS]; break; }
@@ -599,8 +684,10 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
   }
   launch_pcg_finish(s, y, P.p, n3, sc, fl);
   T3X_HIP(hipMemcpyAsync(P.hbuf, sc, SC_COUNT_PCG * sizeof(double), hipMemcpyDeviceToHost, s));
+  T3X_HIP(hipMemcpyAsync(P.hbuf + 32, fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
   P.sync();
   if (resOut) *resOut = P.hbuf[SC_RES];
+  P.lastReliable = ((const int*)(P.hbuf + 32))[FL_NREL];
   if (warned) *warned = !done;  // 634-637
   return its;
 }
@@ -645,7 +732,7 @@ int filter_run(t3x_problem& P, int slot, const double* src, double* dst) {
   const Geo g0 = geo(P.lv[0]);
   const long long NN = P.NN();
   double* sc = P.sc + 16;
-  int* fl = P.fl + 4;
+  int* fl = P.fl + 8;
   const double* kern = P.dFiltKern + slot * 64;
   const double* dinv = P.filt[slot].diagInv;
   launch_filter_e2n(s, g0, P.lv[0].emask, src, P.fb);  // 880-886
@@ -1005,6 +1092,11 @@ int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, co
     *out = P.release();
     return T3X_ERR_INPUT;
   }
+  if (P->P.precision < 0 || P->P.precision > 2) {
+    P->err = "params.precision must be 0 (FP64), 1 (mixed) or 2 (FP32 V-cycle only)";
+    *out = P.release();
+    return T3X_ERR_INPUT;
+  }
   int rc;
   try {
     rc = setup(*P, voxels, nFix, fixNodes, fixState, nLoad, loadNodes, loadVec);
@@ -1103,7 +1195,12 @@ int t3x_vcycle(t3x_problem* h, const double* r, double* x) {
   return guarded(h, [&] {
     if (!h->assembled) { h->err = "t3x_vcycle before t3x_assemble"; return (int)T3X_ERR_INPUT; }
     upload_nodal(*h, r, h->r);
-    vcycle(*h, h->r, h->z, h->sc, nullptr, false);
+    if (h->mixed()) {
+      vcycle_mixed(*h, h->r, h->z32, h->sc, nullptr, false);
+      launch_convert(h->stream, h->z, h->z32, 3 * h->NN());
+    } else {
+      vcycle(*h, h->r, h->z, h->sc, nullptr, false);
+    }
     download_nodal(*h, h->z, x);
     return (int)T3X_OK;
   });

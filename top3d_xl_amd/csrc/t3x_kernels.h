@@ -8,9 +8,14 @@ namespace t3x {
 // Scalar slots in device memory (double sc[] / int fl[]) for the solvers.
 enum ScalarSlot {
   SC_NORMB = 0, SC_ZR, SC_PQ, SC_RR, SC_ZRN, SC_RES, SC_LAMBDA, SC_BETA,
-  SC_TOL, SC_COUNT_PCG  // one PCG block = SC_COUNT_PCG doubles
+  SC_TOL,
+  SC_DELTA,  // reliable-update factor (0: off)
+  SC_RLAST,  // relative residual at the last reliable update (< 0: not yet set)
+  SC_COUNT_PCG  // one PCG block = SC_COUNT_PCG doubles (<= 16)
 };
-enum FlagSlot { FL_DONE = 0, FL_ITS, FL_ITER, FL_COUNT };
+// FL_REL: a reliable update (true FP64 residual) is due this iteration;
+// FL_YCUR: y already holds this iteration's lambda*p.
+enum FlagSlot { FL_DONE = 0, FL_ITS, FL_ITER, FL_REL, FL_YCUR, FL_NREL, FL_COUNT };  // <= 8
 
 // Geometry passed by value to kernels.
 struct Geo {
@@ -26,13 +31,21 @@ struct RedWs {
   unsigned int* counter;
 };
 
+// One-block finalisers: fold nb partials in a fixed order, then apply the
+// solver's scalar step (CG step lengths, convergence flag).
+enum FinOp { FIN_PQ = 0, FIN_ZR_INIT, FIN_ZR_LOOP, FIN_RR, FIN_SUM, FIN_SQRT };
+void launch_fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out);
+
 // ---------------------------------------------------------------- FEM
 // tiled K*u (t3x_applyk.hip); partials: one per work-group (nullptr: none)
 int applyk_tiled_blocks(const Geo& g);
 void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                          const double* u, double* y, double* partials, const int* fl);
-void launch_apply_K_tiled_f32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                              const float* u, float* y, double* partials, const int* fl);
+                          const double* u, double* y, double* partials, const int* fl, int gate = -1);
+// mixed precision: FP64 input vector, FP32 Ke/E/arithmetic/output, FP64 u.y partials
+void launch_apply_K_tiled_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                                const double* u, float* y, double* partials, const int* fl);
+void launch_apply_K_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                          const double* u, float* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                            const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
@@ -51,14 +64,30 @@ void launch_galerkin_l2_bc(hipStream_t s, const Geo& gf, const Geo& gc, int span
 void launch_galerkin_coarse(hipStream_t s, const Geo& gf, const Geo& gc, const double* KsF, const uint8_t* emaskF,
                             const uint8_t* emaskC, double* KsC);
 
-// ---------------------------------------------------------------- MG
-void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, const uint8_t* nmaskC,
-                     double* rc, const int* fl);
+// ---------------------------------------------------------------- MG (t3x_mg.hip)
+// Separable restriction through an intermediate of restrict_tmp_len(gf, gc)
+// elements; prolongation + weighted Jacobi with optional fused z.r.
+long long restrict_tmp_len(const Geo& gf, const Geo& gc);
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, double* tmp,
+                     const uint8_t* nmaskC, double* rc, const int* fl);
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, float* tmp,
+                     const uint8_t* nmaskC, float* rc, const int* fl);
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* rf, float* tmp,
+                     const uint8_t* nmaskC, float* rc, const int* fl);
 void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
                            const double* rf, const double* diag, const uint8_t* nmaskF, double* out,
                            const RedWs* ws, double* sc, const int* fl);
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const float* xc,
+                           const float* rf, const float* diag, const uint8_t* nmaskF, float* out,
+                           const RedWs* ws, double* sc, const int* fl);
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const float* xc,
+                           const double* rf, const float* diag, const uint8_t* nmaskF, float* out,
+                           const RedWs* ws, double* sc, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl);
+void launch_coarse_gather(hipStream_t s, int n, const long long* map, const float* r, double* v, const int* fl);
 void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long NN3,
+                           const int* fl);
+void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, float* x, long long NN3,
                            const int* fl);
 void launch_band_assemble(hipStream_t s, const Geo& g, const double* Ks, const uint8_t* emask, const int* nodePos,
                           int bw, double* band, long long nBand);
@@ -78,7 +107,18 @@ void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, con
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
                         const int* fl);
 void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl);
-void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol);
+// mixed precision: FP32 q = K p and z = M r, FP64 y, r, p
+void launch_pcg_update(hipStream_t s, double* r, const float* q, long long n, const RedWs& ws, double* sc, int* fl);
+void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const float* z, long long n, const double* sc,
+                        const int* fl);
+void launch_convert(hipStream_t s, float* dst, const double* src, long long n);
+void launch_convert(hipStream_t s, double* dst, const float* src, long long n);
+void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol, double delta = 0.0);
+// Reliable update of the mixed PCG, every kernel a no-op unless fl[FL_REL]:
+// y += lambda p; q = K y (FP64); r = b - q; the true residual alone may end the solve.
+void launch_reliable_update(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                            const double* b, double* y, const double* p, double* q, double* r, long long n,
+                            const RedWs& ws, double* sc, int* fl);
 void launch_scale_diag(hipStream_t s, double* z, const double* dinv, const double* r, long long n,
                        const RedWs* ws, double* sc, const int* fl);
 void launch_gather_compact(hipStream_t s, const int* map, int n, int ncomp, long long NN, const double* dense,

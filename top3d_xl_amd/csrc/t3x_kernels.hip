@@ -38,8 +38,6 @@ inline int grid_for(long long n) {
 // One-block finalisers: fold a reduction's partials (fixed order) and apply
 // the scalar logic of the solver step that consumes it.
 // =========================================================================
-enum FinOp { FIN_PQ = 0, FIN_ZR_INIT, FIN_ZR_LOOP, FIN_RR, FIN_SUM, FIN_SQRT };
-
 __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* __restrict__ partials, double* sc,
                                                  int* fl, double* out) {
   if (fl && fl[FL_DONE]) return;
@@ -51,6 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* _
     case FIN_PQ:  // 615
       sc[SC_PQ] = t;
       sc[SC_LAMBDA] = sc[SC_ZR] / t;
+      fl[FL_YCUR] = 0;
       break;
     case FIN_ZR_INIT:  // 612
       sc[SC_ZR] = t;
@@ -66,7 +65,16 @@ __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* _
       sc[SC_RES] = res;
       const int it = fl[FL_ITER] + 1;
       fl[FL_ITER] = it;
-      if (res < sc[SC_TOL]) {
+      const double delta = sc[SC_DELTA];
+      if (delta > 0.0) {
+        // mixed precision: the recursive residual only schedules a true
+        // FP64 residual (launch_reliable_update), which alone may end the solve
+        if (sc[SC_RLAST] < 0.0) sc[SC_RLAST] = res;
+        if (res < sc[SC_TOL] || res < delta * sc[SC_RLAST]) {
+          fl[FL_REL] = 1;
+          fl[FL_YCUR] = 1;
+        }
+      } else if (res < sc[SC_TOL]) {
         fl[FL_DONE] = 1;
         fl[FL_ITS] = it;
       }
@@ -76,8 +84,11 @@ __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* _
     default: break;
   }
 }
-static void fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out) {
+void launch_fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out) {
   hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBlock), 0, s, op, nb, ws.partials, sc, fl, out);
+}
+static void fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out) {
+  launch_fin1(s, op, nb, ws, sc, fl, out);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fin_max(int nb, const double* __restrict__ partials, double* out) {
@@ -159,6 +170,11 @@ void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const 
 void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                     const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
   launch_apply_K_tiled(s, g, Ke, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
+  if (pcgStep) fin1(s, FIN_PQ, applyk_tiled_blocks(g), ws, sc, (int*)fl, nullptr);
+}
+void launch_apply_K_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                          const double* u, float* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
+  launch_apply_K_tiled_mixed(s, g, Ke, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
   if (pcgStep) fin1(s, FIN_PQ, applyk_tiled_blocks(g), ws, sc, (int*)fl, nullptr);
 }
 
@@ -448,143 +464,39 @@ void launch_galerkin_coarse(hipStream_t s, const Geo& gf, const Geo& gc, const d
   hipLaunchKernelGGL(k_galerkin_coarse, dim3((unsigned)gc.NN), dim3(64), 0, s, gf, gc, KsF, emaskF, emaskC, KsC);
 }
 
-// =========================================================================
-// Multigrid transfer (783-814) as masked tensor-product stencils.
-// R = P^T with P the trilinear hat interpolation; the reference's division
-// by transferMatCoeffi and solid-node selection reduce to exactly this.
-// =========================================================================
-__global__ __launch_bounds__(kBlock) void k_restrict(Geo gf, Geo gc, int span, const double* __restrict__ rf,
-                                                     const uint8_t* __restrict__ nmaskC, double* __restrict__ rc,
-                                                     const int* fl) {
-  if (fl && fl[FL_DONE]) return;
-  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= gc.NN) return;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  if (nmaskC[n]) {
-    int Jy, Jx, Jz;
-    node_yxz(n, gc, Jy, Jx, Jz);
-    const int y0 = span * Jy, x0 = span * Jx, z0 = span * Jz;
-    const double inv = 1.0 / span;
-    for (int dz = -(span - 1); dz <= span - 1; ++dz) {
-      const int z = z0 + dz;
-      if (z < 0 || z >= gf.NZ) continue;
-      const double wz = (span - abs(dz)) * inv;
-      for (int dx = -(span - 1); dx <= span - 1; ++dx) {
-        const int x = x0 + dx;
-        if (x < 0 || x >= gf.NX) continue;
-        const double wzx = wz * ((span - abs(dx)) * inv);
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-        for (int dy = -(span - 1); dy <= span - 1; ++dy) {
-          const int y = y0 + dy;
-          if (y < 0 || y >= gf.NY) continue;
-          const double wy = (span - abs(dy)) * inv;
-          const long long f = nidx(gf, y, x, z);
-          t0 += wy * rf[f];
-          t1 += wy * rf[gf.NN + f];
-          t2 += wy * rf[2 * gf.NN + f];
-        }
-        a0 += wzx * t0;
-        a1 += wzx * t1;
-        a2 += wzx * t2;
-      }
-    }
-  }
-  rc[n] = a0;
-  rc[gc.NN + n] = a1;
-  rc[2 * gc.NN + n] = a2;
-}
-
-void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, const uint8_t* nmaskC,
-                     double* rc, const int* fl) {
-  hipLaunchKernelGGL(k_restrict, dim3(grid_for(gc.NN)), dim3(kBlock), 0, s, gf, gc, span, rf, nmaskC, rc, fl);
-}
-
-// out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere
-// (758, 770, 773).  Optionally fuses the z.r reduction of the PCG (629).
-__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(Geo gf, Geo gc, int span, double omega,
-                                                           const double* __restrict__ xc,
-                                                           const double* __restrict__ rf,
-                                                           const double* __restrict__ diag,
-                                                           const uint8_t* __restrict__ nmaskF,
-                                                           double* __restrict__ out, RedWs ws, double* sc,
-                                                           const int* fl, int mode) {
-  if (fl && fl[FL_DONE]) return;
-  const long long NN = gf.NN;
-  double zr = 0.0;
-  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < NN; n += (long long)gridDim.x * kBlock) {
-    double o[3] = {0.0, 0.0, 0.0};
-    if (nmaskF[n]) {
-      int y, x, z;
-      node_yxz(n, gf, y, x, z);
-      const int Jy = y / span, Jx = x / span, Jz = z / span;
-      const int ty = y - span * Jy, tx = x - span * Jx, tz = z - span * Jz;
-      const double inv = 1.0 / span;
-      const double wy[2] = {(span - ty) * inv, ty * inv};
-      const double wx[2] = {(span - tx) * inv, tx * inv};
-      const double wz[2] = {(span - tz) * inv, tz * inv};
-      double p[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
-        const double w = wy[a] * wx[b] * wz[d];
-        if (w == 0.0) continue;
-        const long long J = nidx(gc, Jy + a, Jx + b, Jz + d);
-        p[0] += w * xc[J];
-        p[1] += w * xc[gc.NN + J];
-        p[2] += w * xc[2 * gc.NN + J];
-      }
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double r = rf[k * NN + n], dd = diag[k * NN + n];
-        o[k] = (omega * (r / dd) + p[k]) + (omega * r) / dd;
-        zr += o[k] * r;
-      }
-    }
-    out[n] = o[0];
-    out[NN + n] = o[1];
-    out[2 * NN + n] = o[2];
-  }
-  if (mode > 0) {
-    double v[1] = {zr};
-    write_partials<1>(v, ws.partials);
-  }
-}
-
-void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
-                           const double* rf, const double* diag, const uint8_t* nmaskF, double* out,
-                           const RedWs* ws, double* sc, const int* fl) {
-  // ws != nullptr: fused reduction; mode 1 = initial z.r, mode 2 = in-loop
-  int mode = 0;
-  RedWs w{nullptr, nullptr};
-  if (ws) { w = *ws; mode = (fl && sc) ? 2 : 1; }
-  const int nb = mode ? red_blocks(gf.NN) : grid_for(gf.NN);
-  hipLaunchKernelGGL(k_prolong_jacobi, dim3(nb), dim3(kBlock), 0, s, gf, gc, span, omega, xc, rf, diag, nmaskF, out,
-                     w, sc, fl, mode);
-  if (mode) fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, nb, w, sc, (int*)fl, nullptr);
-}
+// Multigrid transfers (783-814) live in t3x_mg.hip.
 
 // =========================================================================
 // Coarsest level: banded Cholesky (1052-1065) and solves (767).
 // Band storage: row i holds L(i, i-k) at band[i*W + k], W = bw+1.
 // =========================================================================
-__global__ void k_coarse_gather(int n, const long long* __restrict__ map, const double* __restrict__ r,
+template <typename TR>
+__global__ void k_coarse_gather(int n, const long long* __restrict__ map, const TR* __restrict__ r,
                                 double* __restrict__ v, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) v[i] = r[map[i]];
+  if (i < n) v[i] = (double)r[map[i]];
 }
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl) {
-  hipLaunchKernelGGL(k_coarse_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, r, v, fl);
+  hipLaunchKernelGGL(k_coarse_gather<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, r, v, fl);
 }
+void launch_coarse_gather(hipStream_t s, int n, const long long* map, const float* r, double* v, const int* fl) {
+  hipLaunchKernelGGL(k_coarse_gather<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, r, v, fl);
+}
+template <typename TX>
 __global__ void k_coarse_scatter(int n, const long long* __restrict__ map, const double* __restrict__ v,
-                                 double* __restrict__ x, const int* fl) {
+                                 TX* __restrict__ x, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[map[i]] = v[i];
+  if (i < n) x[map[i]] = (TX)v[i];
 }
 void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long,
                            const int* fl) {
-  hipLaunchKernelGGL(k_coarse_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, v, x, fl);
+  hipLaunchKernelGGL(k_coarse_scatter<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, v, x, fl);
+}
+void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, float* x, long long,
+                           const int* fl) {
+  hipLaunchKernelGGL(k_coarse_scatter<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, map, v, x, fl);
 }
 
 // Node-centric assembly of the lower band: the thread of coarsest node n owns
@@ -842,7 +754,7 @@ void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, lo
 // y += lambda p is applied here (no-op if the solve hit maxIT).
 __global__ void k_pcg_finish(double* __restrict__ y, const double* __restrict__ p, long long n, const double* sc,
                              const int* fl) {
-  if (!fl[FL_DONE]) return;
+  if (!fl[FL_DONE] || fl[FL_YCUR]) return;
   const double lambda = sc[SC_LAMBDA];
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
     y[i] = y[i] + lambda * p[i];
@@ -851,14 +763,104 @@ void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, c
   hipLaunchKernelGGL(k_pcg_finish, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, n, sc, fl);
 }
 
-__global__ void k_pcg_init(double* sc, int* fl, double tol) {
-  sc[SC_TOL] = tol;
-  fl[FL_DONE] = 0;
-  fl[FL_ITS] = 0;
-  fl[FL_ITER] = 0;
+// Mixed precision (FP32 q and z, FP64 r, p, y and scalars): same steps.
+__global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(double* __restrict__ r, const float* __restrict__ q,
+                                                             long long n, RedWs ws, double* sc, int* fl) {
+  if (fl[FL_DONE]) return;
+  const double lambda = sc[SC_LAMBDA];
+  double rr = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    const double ri = r[i] - lambda * (double)q[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  double v[1] = {rr};
+  write_partials<1>(v, ws.partials);
 }
-void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol) {
-  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1), 0, s, sc, fl, tol);
+void launch_pcg_update(hipStream_t s, double* r, const float* q, long long n, const RedWs& ws, double* sc, int* fl) {
+  hipLaunchKernelGGL(k_pcg_update_mixed, dim3(red_blocks(n)), dim3(kBlock), 0, s, r, q, n, ws, sc, fl);
+  fin1(s, FIN_RR, red_blocks(n), ws, sc, fl, nullptr);
+}
+__global__ void k_pcg_pupdate_mixed(double* __restrict__ y, double* __restrict__ p, const float* __restrict__ z,
+                                    long long n, const double* sc, const int* fl) {
+  if (fl[FL_DONE]) return;
+  const double beta = sc[SC_BETA], lambda = fl[FL_YCUR] ? 0.0 : sc[SC_LAMBDA];
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    const double pi = p[i];
+    y[i] = y[i] + lambda * pi;
+    p[i] = (double)z[i] + beta * pi;
+  }
+}
+void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const float* z, long long n, const double* sc,
+                        const int* fl) {
+  hipLaunchKernelGGL(k_pcg_pupdate_mixed, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, z, n, sc, fl);
+}
+template <typename TD, typename TS>
+__global__ void k_convert(TD* __restrict__ dst, const TS* __restrict__ src, long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    dst[i] = (TD)src[i];
+}
+void launch_convert(hipStream_t s, float* dst, const double* src, long long n) {
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
+}
+void launch_convert(hipStream_t s, double* dst, const float* src, long long n) {
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
+}
+
+__global__ void k_pcg_init(double* sc, int* fl, double tol, double delta) {
+  sc[SC_TOL] = tol;
+  sc[SC_DELTA] = delta;
+  sc[SC_RLAST] = -1.0;
+  for (int i = 0; i < FL_COUNT; ++i) fl[i] = 0;
+}
+void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol, double delta) {
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1), 0, s, sc, fl, tol, delta);
+}
+
+__global__ void k_rel_axpy(double* __restrict__ y, const double* __restrict__ p, long long n, const double* sc,
+                           const int* fl) {
+  if (fl[FL_DONE] || !fl[FL_REL]) return;
+  const double lambda = sc[SC_LAMBDA];
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    y[i] = y[i] + lambda * p[i];
+}
+__global__ __launch_bounds__(kBlock) void k_rel_residual(double* __restrict__ r, const double* __restrict__ b,
+                                                         const double* __restrict__ q, long long n, RedWs ws,
+                                                         const int* fl) {
+  if (fl[FL_DONE] || !fl[FL_REL]) return;
+  double rr = 0.0;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    const double ri = b[i] - q[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  double v[1] = {rr};
+  write_partials<1>(v, ws.partials);
+}
+__global__ __launch_bounds__(kBlock) void k_fin_rrt(int nb, const double* __restrict__ partials, double* sc, int* fl) {
+  if (fl[FL_DONE] || !fl[FL_REL]) return;
+  double v[1];
+  fold_partials<1>(partials, nb, v);
+  if (threadIdx.x != 0) return;
+  const double t = v[0];
+  const double res = sqrt(t) / sc[SC_NORMB];
+  sc[SC_RR] = t;
+  sc[SC_RES] = res;
+  sc[SC_RLAST] = res;
+  fl[FL_REL] = 0;
+  fl[FL_NREL] += 1;
+  if (res < sc[SC_TOL]) {
+    fl[FL_DONE] = 1;
+    fl[FL_ITS] = fl[FL_ITER];
+  }
+}
+void launch_reliable_update(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                            const double* b, double* y, const double* p, double* q, double* r, long long n,
+                            const RedWs& ws, double* sc, int* fl) {
+  hipLaunchKernelGGL(k_rel_axpy, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, n, sc, fl);
+  launch_apply_K_tiled(s, g, Ke, E, fix, y, q, nullptr, fl, FL_REL);
+  hipLaunchKernelGGL(k_rel_residual, dim3(red_blocks(n)), dim3(kBlock), 0, s, r, b, q, n, ws, fl);
+  hipLaunchKernelGGL(k_fin_rrt, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, sc, fl);
 }
 
 // z = dinv .* r (889) with the fused z.r reduction.

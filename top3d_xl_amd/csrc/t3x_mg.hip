@@ -1,0 +1,240 @@
+// Multigrid transfers of the 'Adapted' V-cycle (TOP3D_XL.m:747-814) for gfx950.
+//
+// R = P^T with P the trilinear hat interpolation over the solid region; the
+// reference's scatter into intermediate nodes, division by transferMatCoeffi
+// and solid-node selection (783-814) reduce to masked tensor-product
+// stencils with 1-D weights (S-|d|)/S, S the span (4 for the first
+// coarsening, 2 after).  Vectors are zero outside the solid region, so only
+// the coarse mask is applied.
+//
+// Restriction is separable: a y pass (fine -> NYc x NXf x NZf) and an x-z
+// pass (-> coarse), each thread owning one output, a wave reading
+// consecutive y.  The y pass reads every fine value once from HBM; the
+// intermediate is 1/S of the fine vector.  Prolongation + Jacobi runs one
+// thread per (y, x) column walking z, so the coarse-column weights are
+// computed once per thread and no index division happens in the z loop.
+//
+// Templates over the vector types give the FP64 path (double everywhere) and
+// the mixed path (FP64 fine residual, FP32 coarse levels and smoother
+// output) from the same code.
+#include <hip/hip_runtime.h>
+
+#include "t3x_kernels.h"
+#include "t3x_reduce.h"
+
+namespace t3x {
+
+namespace {
+
+template <int S>
+__device__ __forceinline__ double hat_w(int d) {
+  return (double)(S - (d < 0 ? -d : d)) * (1.0 / S);
+}
+
+// y pass: tmp[c][z][x][Jy] = sum_dy w(dy) rf[c](S*Jy+dy, x, z)
+template <int S, typename TI, typename TM>
+__global__ __launch_bounds__(kBlock) void k_restrict_y(Geo gf, int NYc, const TI* __restrict__ rf,
+                                                       TM* __restrict__ tmp, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int plane = NYc * gf.NX;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= plane) return;
+  const int x = idx / NYc, Jy = idx - x * NYc;
+  const int z = blockIdx.y;
+  const long long base = (long long)gf.NY * ((long long)x + (long long)gf.NX * z);
+  const long long TN = (long long)plane * gf.NZ;
+  const long long o = idx + (long long)plane * z;
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+  for (int dy = -(S - 1); dy <= S - 1; ++dy) {
+    const int y = S * Jy + dy;
+    if (y < 0 || y >= gf.NY) continue;
+    const double w = hat_w<S>(dy);
+    const long long f = base + y;
+    t0 += w * (double)rf[f];
+    t1 += w * (double)rf[gf.NN + f];
+    t2 += w * (double)rf[2 * gf.NN + f];
+  }
+  tmp[o] = (TM)t0;
+  tmp[TN + o] = (TM)t1;
+  tmp[2 * TN + o] = (TM)t2;
+}
+
+// x-z pass: rc[c](J) = mask(J) * sum_dz w(dz) sum_dx w(dx) tmp[c](Jy, S*Jx+dx, S*Jz+dz)
+template <int S, typename TM, typename TO>
+__global__ __launch_bounds__(kBlock) void k_restrict_xz(Geo gf, Geo gc, const TM* __restrict__ tmp,
+                                                        const uint8_t* __restrict__ nmaskC, TO* __restrict__ rc,
+                                                        const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int planeC = gc.NY * gc.NX;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= planeC) return;
+  const int Jx = idx / gc.NY, Jy = idx - Jx * gc.NY;
+  const int Jz = blockIdx.y;
+  const long long n = idx + (long long)planeC * Jz;
+  const int plane = gc.NY * gf.NX;  // intermediate plane: NYc x NXf
+  const long long TN = (long long)plane * gf.NZ;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  if (nmaskC[n]) {
+#pragma unroll
+    for (int dz = -(S - 1); dz <= S - 1; ++dz) {
+      const int z = S * Jz + dz;
+      if (z < 0 || z >= gf.NZ) continue;
+      const double wz = hat_w<S>(dz);
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int dx = -(S - 1); dx <= S - 1; ++dx) {
+        const int x = S * Jx + dx;
+        if (x < 0 || x >= gf.NX) continue;
+        const double wx = hat_w<S>(dx);
+        const long long m = Jy + (long long)gc.NY * x + (long long)plane * z;
+        s0 += wx * (double)tmp[m];
+        s1 += wx * (double)tmp[TN + m];
+        s2 += wx * (double)tmp[2 * TN + m];
+      }
+      a0 += wz * s0;
+      a1 += wz * s1;
+      a2 += wz * s2;
+    }
+  }
+  rc[n] = (TO)a0;
+  rc[gc.NN + n] = (TO)a1;
+  rc[2 * gc.NN + n] = (TO)a2;
+}
+
+// out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere (758,
+// 770, 773); optional fused z.r partials (629).
+template <int S, typename TX, typename TR, typename TD, typename TO>
+__global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega, const TX* __restrict__ xc,
+                                                    const TR* __restrict__ rf, const TD* __restrict__ diag,
+                                                    const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
+                                                    double* __restrict__ partials, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int plane = gf.NY * gf.NX;
+  const long long NN = gf.NN, NC = gc.NN;
+  const long long planeC = (long long)gc.NY * gc.NX;
+  double zr = 0.0;
+  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < plane; idx += gridDim.x * kBlock) {
+    const int x = idx / gf.NY, y = idx - x * gf.NY;
+    const int Jy = y / S, Jx = x / S;
+    const int ty = y - S * Jy, tx = x - S * Jx;
+    const double wy[2] = {(double)(S - ty) * (1.0 / S), (double)ty * (1.0 / S)};
+    const double wx[2] = {(double)(S - tx) * (1.0 / S), (double)tx * (1.0 / S)};
+    // clamped neighbours: a zero weight never reads past the last coarse node
+    const int Jy1 = min(Jy + 1, gc.NY - 1), Jx1 = min(Jx + 1, gc.NX - 1);
+    const long long cyx[4] = {Jy + (long long)gc.NY * Jx, Jy1 + (long long)gc.NY * Jx, Jy + (long long)gc.NY * Jx1,
+                              Jy1 + (long long)gc.NY * Jx1};
+    for (int z = blockIdx.y; z < gf.NZ; z += gridDim.y) {
+      const long long n = idx + (long long)plane * z;
+      double o[3] = {0.0, 0.0, 0.0};
+      if (nmaskF[n]) {
+        const int Jz = z / S, tz = z - S * Jz;
+        const int Jz1 = min(Jz + 1, gc.NZ - 1);
+        const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
+        const long long zo[2] = {planeC * Jz, planeC * Jz1};
+        double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
+          const double w = wy[a] * wx[b] * wz[d];
+          const long long J = cyx[a + 2 * b] + zo[d];
+          p[0] += w * (double)xc[J];
+          p[1] += w * (double)xc[NC + J];
+          p[2] += w * (double)xc[2 * NC + J];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double r = (double)rf[k * NN + n], dd = (double)diag[k * NN + n];
+          const TO v = (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd);
+          o[k] = (double)v;
+          zr += o[k] * r;
+        }
+      }
+      out[n] = (TO)o[0];
+      out[NN + n] = (TO)o[1];
+      out[2 * NN + n] = (TO)o[2];
+    }
+  }
+  if (partials) {
+    double v[1] = {zr};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x + (size_t)gridDim.x * blockIdx.y] = v[0];
+  }
+}
+
+template <typename TI, typename TM, typename TO>
+void restrict_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const TI* rf, TM* tmp,
+                   const uint8_t* nmaskC, TO* rc, const int* fl) {
+  const int plane = gc.NY * gf.NX;
+  const dim3 g1((plane + kBlock - 1) / kBlock, gf.NZ);
+  const dim3 g2((gc.NY * gc.NX + kBlock - 1) / kBlock, gc.NZ);
+  if (span == 4) {
+    hipLaunchKernelGGL((k_restrict_y<4, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl);
+    hipLaunchKernelGGL((k_restrict_xz<4, TM, TO>), g2, dim3(kBlock), 0, s, gf, gc, tmp, nmaskC, rc, fl);
+  } else {
+    hipLaunchKernelGGL((k_restrict_y<2, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl);
+    hipLaunchKernelGGL((k_restrict_xz<2, TM, TO>), g2, dim3(kBlock), 0, s, gf, gc, tmp, nmaskC, rc, fl);
+  }
+}
+
+dim3 prolong_grid(const Geo& gf, bool reduce) {
+  const int plane = gf.NY * gf.NX;
+  int bx = (plane + kBlock - 1) / kBlock;
+  if (!reduce) return dim3(bx, gf.NZ);
+  if (bx > kMaxRedBlocks) bx = kMaxRedBlocks;
+  int by = kMaxRedBlocks / bx;
+  if (by > gf.NZ) by = gf.NZ;
+  if (by < 1) by = 1;
+  return dim3(bx, by);
+}
+
+template <typename TX, typename TR, typename TD, typename TO>
+void prolong_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const TX* xc, const TR* rf,
+                  const TD* diag, const uint8_t* nmaskF, TO* out, const RedWs* ws, double* sc, const int* fl) {
+  int mode = 0;
+  if (ws) mode = (fl && sc) ? 2 : 1;
+  const dim3 g = prolong_grid(gf, mode != 0);
+  double* part = mode ? ws->partials : nullptr;
+  if (span == 4)
+    hipLaunchKernelGGL((k_prolong<4, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, xc, rf, diag, nmaskF,
+                       out, part, fl);
+  else
+    hipLaunchKernelGGL((k_prolong<2, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, xc, rf, diag, nmaskF,
+                       out, part, fl);
+  if (mode) launch_fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, (int)(g.x * g.y), *ws, sc, (int*)fl, nullptr);
+}
+
+}  // namespace
+
+long long restrict_tmp_len(const Geo& gf, const Geo& gc) { return 3LL * gc.NY * gf.NX * gf.NZ; }
+
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, double* tmp,
+                     const uint8_t* nmaskC, double* rc, const int* fl) {
+  restrict_impl<double, double, double>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl);
+}
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, float* tmp,
+                     const uint8_t* nmaskC, float* rc, const int* fl) {
+  restrict_impl<double, float, float>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl);
+}
+void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* rf, float* tmp,
+                     const uint8_t* nmaskC, float* rc, const int* fl) {
+  restrict_impl<float, float, float>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl);
+}
+
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
+                           const double* rf, const double* diag, const uint8_t* nmaskF, double* out,
+                           const RedWs* ws, double* sc, const int* fl) {
+  prolong_impl(s, gf, gc, span, omega, xc, rf, diag, nmaskF, out, ws, sc, fl);
+}
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const float* xc,
+                           const float* rf, const float* diag, const uint8_t* nmaskF, float* out,
+                           const RedWs* ws, double* sc, const int* fl) {
+  prolong_impl(s, gf, gc, span, omega, xc, rf, diag, nmaskF, out, ws, sc, fl);
+}
+void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const float* xc,
+                           const double* rf, const float* diag, const uint8_t* nmaskF, float* out,
+                           const RedWs* ws, double* sc, const int* fl) {
+  prolong_impl(s, gf, gc, span, omega, xc, rf, diag, nmaskF, out, ws, sc, fl);
+}
+
+}  // namespace t3x
