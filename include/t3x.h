@@ -46,12 +46,15 @@ typedef struct t3x_params {
   double omega;          /* weightFactorJacobi_ (32) 0.6 */
   int coarsestControl;   /* coarsestResolutionControl_ (33) 50000 */
   int precision;         /* 0 = FP64 plain path;
-                            1 = mixed: FP64 CG vectors, dots and step lengths,
-                                FP32 K*p and FP32 V-cycle levels, FP64 coarsest
-                                Cholesky, FP64 reliable updates of the residual;
-                            2 = FP64 CG and K*p with the FP32 V-cycle only */
+                            1 = mixed: FP64 CG (K*p, vectors, dots, step
+                                lengths) preconditioned by the V-cycle with FP32
+                                level vectors, diagonals and smoother output
+                                around the FP64 coarsest Cholesky */
   int checkEvery;        /* PCG convergence flag read-back interval (>=1) */
   int timing;            /* 1 = time kernel groups with HIP events (t3x_kernel_time) */
+  int vcycle;            /* typeVcycle_ (34): 0 = 'Adapted' (747-781); 1 = 'Standard'
+                            (V(1,1) damped Jacobi with K*u on every level, 759-777),
+                            FP32, available with precision = 1 */
 } t3x_params;
 
 /* Fills the reference defaults (InitialSettings, 21-38). */
@@ -128,6 +131,20 @@ int t3x_opt_result(t3x_problem* h, t3x_result* out);
 int t3x_kernel_time(t3x_problem* h, int group, long long* count, double* ms);
 int t3x_kernel_time_reset(t3x_problem* h);
 
+/* Device bytes a problem of this domain allocates over a whole run (an upper
+ * bound: coarsest band at the full-box bandwidth); voxels may be NULL for a
+ * full cuboid.  No counterpart in the reference: the admission rule of the
+ * serving engine (t3x_serve.h). */
+int t3x_footprint(const uint8_t* voxels, int nely, int nelx, int nelz, const t3x_params* params, int mode,
+                  long long* bytes);
+/* Device bytes the handle holds now.  No counterpart in the reference. */
+int t3x_memory(const t3x_problem* h, long long* bytes);
+/* Times reps device-to-device copies of `bytes` bytes by this library's
+ * streaming-copy kernel (HIP events); *msPerCopy per copy.  No counterpart in
+ * the reference: the practical-bandwidth and byte-counter calibration
+ * reference of the benchmark. */
+int t3x_stream_copy(long long bytes, int reps, double* msPerCopy);
+
 /* ---- step-level entry points (compact host buffers) ---------------------- */
 /* meshHierarchy_(1).eleModulus = E (116) */
 int t3x_set_modulus(t3x_problem* h, const double* E);
@@ -139,11 +156,7 @@ int t3x_apply_K(t3x_problem* h, const double* u, double* y);
  * V-cycle (FP64 coarsest Cholesky) */
 int t3x_vcycle(t3x_problem* h, const double* r, double* x);
 /* Solving_PCG(@Solving_KbyU_MatrixFree, @Solving_Vcycle, ...) (600-639); x holds x0 on entry.
- * With params.precision = 1: FP64 y, r, p, dot products and step lengths,
- * FP32 K*p (FP64 input, FP32 arithmetic and output) and FP32 V-cycle; the
- * recursive residual is replaced by the true FP64 residual b - K*y whenever
- * it has dropped tenfold and before the solve may stop, so the stopping
- * test always sees the true residual. */
+ * With params.precision = 1 the preconditioner is the FP32 V-cycle. */
 int t3x_pcg(t3x_problem* h, const double* b, double* x, double tol, int maxIT, int* its, double* resnorm);
 /* TopOpti_ComputeUnitCompliance (579-598) */
 int t3x_compliance(t3x_problem* h, const double* U, double* ce);

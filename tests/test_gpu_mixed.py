@@ -1,17 +1,23 @@
 """Mixed-precision form (params.precision = 1) against the FP64 oracle.
 
-The mixed path keeps the outer CG in FP64 (y, r, p, dot products, step
-lengths) and the coarsest Cholesky in FP64; K*p (FP64 input, FP32 arithmetic
-and output) and the V-cycle levels run in FP32.  Its bar is the compliance
-error against the FP64 oracle, <= 1e-6 relative, for every solve.
+The mixed form is the reference's PCG in FP64 (K*p, y, r, p, dot products,
+step lengths) preconditioned by the V-cycle with FP32 level vectors,
+diagonals and smoother output around the FP64 coarsest Cholesky.  Its bar is
+the compliance error against the FP64 oracle, <= 1e-6 relative, per solve.
 
-Why that bar is meaningful at the reference's CG tolerance: a solve stopped
-at ||r||/||b|| < 1e-3 has a compliance within 3e-9 (uniform design) to 3e-8
-(random 0.05..0.95 design) of the exact one on the 50x100x50 cuboid
-(measured with the oracle at tol 1e-3 vs 1e-10), so two solves that both
-reach the tolerance agree to ~1e-7 whatever their precision; the test also
-checks that the mixed solve's TRUE residual, recomputed by the FP64 oracle,
-meets the tolerance (FP32 rounding in K*p must not stall the FP64 outer CG).
+Why the bar holds with an FP32 preconditioner: CG iterates with the exact
+(FP64) operator are Galerkin, so their compliance error is quadratic in the
+energy error -- a solve stopped at ||r||/||b|| < 1e-3 is within 3e-9
+(uniform design) to 3e-8 (random 0.05..0.95) of the exact compliance on the
+50x100x50 cuboid (oracle at tol 1e-3 vs 1e-10).  A perturbed preconditioner
+changes which Krylov iterate is returned, not the operator.  FP32 K*p, in
+contrast, perturbs the operator itself: on a 256x512x256 grid the smooth
+modes cancel to ~(pi/256)^2 of |K||u| and FP32 K*p stalled the FP64 outer
+CG (800 iterations, residual 1.6 for the uniform design), so K*p stays FP64.
+
+When the two solves stop at different iteration counts (the preconditioner
+differs in rounding), the compliance change the FP64 oracle itself shows
+between those stopping points is admitted on top of the 1e-6 bar.
 """
 import math
 
@@ -31,7 +37,10 @@ CASES = {
 C_TOL = 1e-6
 
 
-PRECISIONS = {"mixed": 1, "fp32vcycle": 2}
+# the mixed form with the reference's 'Adapted' cycle, and with the 'Standard'
+# V(1,1) cycle (759-777) at omega = 0.4: with the reference's 0.6 the damped
+# Jacobi smoother diverged on high-contrast designs (CPU experiment, 24x48x24)
+PRECISIONS = {"mixed": dict(precision=1), "mixed_std": dict(precision=1, vcycle=1, omega=0.4)}
 
 
 @pytest.fixture(scope="module", params=[(c, p) for p in PRECISIONS for c in CASES], ids=lambda v: f"{v[0]}-{v[1]}")
@@ -41,15 +50,26 @@ def mpair(request):
     print(f"\n[gpu-mixed] case {case} precision {prec}", flush=True)
     vox, fix, load = CASES[case]()
     if fix is None:
-        g, o = t3x.Problem(vox, precision=PRECISIONS[prec]), Oracle(vox)
+        g, o = t3x.Problem(vox, **PRECISIONS[prec]), Oracle(vox)
     else:
-        g, o = t3x.Problem(vox, fix, load, precision=PRECISIONS[prec]), Oracle(vox, fix, load)
-    yield dict(g=g, o=o, name=case)
+        g, o = t3x.Problem(vox, fix, load, **PRECISIONS[prec]), Oracle(vox, fix, load)
+    yield dict(g=g, o=o, name=case, prec=prec)
     g.close()
 
 
 def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def stop_bar(o, E, b, x0, its_a, its_b):
+    """max(C_TOL, oracle compliance change between stopping after its_a and its_b)."""
+    if its_a == its_b:
+        return C_TOL
+    cs = []
+    for k in (its_a, its_b):
+        x, _, _ = o.pcg(b, x0=x0, tol=1e-300, maxit=k)
+        cs.append(float(E @ o.compliance(x)))
+    return max(C_TOL, 1.5 * abs(cs[0] - cs[1]) / cs[0])
 
 
 def _set(pair, E):
@@ -59,6 +79,8 @@ def _set(pair, E):
 
 
 def test_mixed_vcycle(mpair):
+    if mpair["prec"] != "mixed":
+        pytest.skip("the oracle restates the 'Adapted' cycle only")
     g, o = mpair["g"], mpair["o"]
     rng = np.random.default_rng(5)
     _set(mpair, 1e-6 + rng.random(o.nE) ** 3)
@@ -84,11 +106,15 @@ def test_mixed_pcg_compliance(mpair, design):
     c_o = float(E @ o.compliance(xo))
     c_g = float(E @ o.compliance(xg))
     true_res = np.linalg.norm(b - o.apply_K(xg)) / np.linalg.norm(b)
-    print(f"  {design}: its fp64 {ito} mixed {itg}; c rel err {abs(c_g - c_o) / c_o:.2e}; "
+    bar = stop_bar(o, E, b, None, ito, itg)
+    print(f"  {design}: its fp64 {ito} mixed {itg}; c rel err {abs(c_g - c_o) / c_o:.2e} (bar {bar:.1e}); "
           f"true res {true_res:.2e} (recursive {res_g:.2e})")
-    assert abs(c_g - c_o) / c_o <= C_TOL
-    assert true_res < 1.05e-3
-    assert abs(itg - ito) <= 2
+    assert abs(c_g - c_o) / c_o <= bar
+    assert true_res < 1e-3
+    if mpair["prec"] == "mixed":
+        assert abs(itg - ito) <= max(2, ito // 10)
+    else:  # a stronger preconditioner: fewer iterations
+        assert itg < ito
 
 
 def test_mixed_warm_start(mpair):
@@ -106,9 +132,11 @@ def test_mixed_warm_start(mpair):
     xg, itg, _ = g.pcg(b, x0=x0)
     c_o = float(E2 @ o.compliance(xo))
     c_g = float(E2 @ o.compliance(xg))
-    print(f"  warm start: its {ito} / {itg}, c rel err {abs(c_g - c_o) / c_o:.2e}")
-    assert abs(c_g - c_o) / c_o <= C_TOL
-    assert abs(itg - ito) <= 2
+    bar = stop_bar(o, E2, b, x0, ito, itg)
+    print(f"  warm start: its {ito} / {itg}, c rel err {abs(c_g - c_o) / c_o:.2e} (bar {bar:.1e})")
+    assert abs(c_g - c_o) / c_o <= bar
+    if mpair["prec"] == "mixed":
+        assert abs(itg - ito) <= 2
 
 
 @pytest.mark.parametrize("prec", list(PRECISIONS))
@@ -121,7 +149,7 @@ def test_mixed_design_loop_first_iterations(prec):
     vox = np.ones((24, 48, 24), bool)
     o = Oracle(vox)
     ro = o.run("GLOBAL", 0.12, 4, math.sqrt(3))
-    g = t3x.Problem(vox, precision=PRECISIONS[prec])
+    g = t3x.Problem(vox, **PRECISIONS[prec])
     rg = g.run("GLOBAL", 0.12, 4, math.sqrt(3))
     g.close()
     errs = np.abs(rg["cHist"] - ro["cHist"]) / ro["cHist"]
@@ -130,3 +158,44 @@ def test_mixed_design_loop_first_iterations(prec):
     assert abs(rg["cSolid"] - ro["cSolid"]) / ro["cSolid"] <= C_TOL
     assert errs[0] <= C_TOL
     assert np.all(errs <= 1e-4)
+
+
+def test_standard_vcycle_matches_numpy():
+    """The FP32 'Standard' cycle against a float64 numpy V(1,1) on the brute-force
+    Galerkin hierarchy (tests/bruteforce.py; independent of the oracle)."""
+    import scipy.sparse.linalg as spla
+    import top3d_xl_amd as t3x
+    from tests.bruteforce import Grid
+    vox = np.ones((8, 16, 8), bool)
+    G = Grid(vox)
+    rng = np.random.default_rng(2)
+    nE = len(G.elems[0])
+    E = 1e-6 + (0.05 + 0.9 * rng.random(nE)) ** 3
+    A, P = G.hierarchy(E)
+    L = G.nlev
+    D = [A[l].diagonal() for l in range(L)]
+    lu = spla.splu(A[L - 1].tocsc())
+    om = 0.4
+
+    def standard(r):
+        rs, xs = [r], [None] * L
+        for l in range(L - 1):
+            xs[l] = om * rs[l] / D[l]
+            rs.append(P[l + 1].T @ (rs[l] - A[l] @ xs[l]))
+        xs[L - 1] = lu.solve(rs[L - 1])
+        for l in range(L - 2, -1, -1):
+            xs[l] = xs[l] + P[l + 1] @ xs[l + 1]
+            xs[l] = xs[l] + om * (rs[l] - A[l] @ xs[l]) / D[l]
+        return xs[0]
+
+    g = t3x.Problem(vox, precision=1, vcycle=1, omega=om)
+    g.set_modulus(E)
+    g.assemble()
+    for _ in range(2):
+        r = rng.standard_normal(3 * G.nN[0])
+        r[G.fixed] = 0.0  # PCG residuals vanish on fixed DOFs
+        zg, zn = g.vcycle(r), standard(r)
+        d = np.abs(zg - zn).max() / np.abs(zn).max()
+        print(f"  standard vcycle rel {d:.2e}")
+        assert d < 1e-4  # FP32 smoothing and transfers around the FP64 coarsest solve
+    g.close()

@@ -56,13 +56,20 @@ __device__ __forceinline__ void cell_pair(const T* __restrict__ Ke, const T (&n0
   }
 }
 
-// T: arithmetic and LDS type; TE, TU, TY: types of E, u and y in HBM.
-template <typename T, typename TE, typename TU, typename TY>
+// Epilogues at a completed node (Ku: the BC operator's row, u on fixed rows):
+//   EPI_KU     y = Ku                         partials: u.y (the PCG's p.q)
+//   EPI_RESID  y = r - Ku                     (the 'Standard' cycle's defect, 761)
+//   EPI_SMOOTH y = u + omega (r - Ku) / d     partials: y.r (post-smoothing, 775)
+enum { EPI_KU = 0, EPI_RESID = 1, EPI_SMOOTH = 2 };
+
+// T: arithmetic and LDS type; TE, TU, TY, TR, TD: types of E, u, y, r, d in HBM.
+template <typename T, typename TE, typename TU, typename TY, int EPI, typename TR, typename TD>
 __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restrict__ Ke, const TE* __restrict__ E,
                                                         const uint8_t* __restrict__ fix, const TU* __restrict__ u,
                                                         TY* __restrict__ y, double* __restrict__ partials,
-                                                        const int* fl, int reduce, int gate) {
-  if (fl && (fl[FL_DONE] || (gate >= 0 && !fl[gate]))) return;
+                                                        const int* fl, int reduce, const TR* __restrict__ rv,
+                                                        const TD* __restrict__ dv, double omega) {
+  if (fl && fl[FL_DONE]) return;
   __shared__ T sU[2][3][AK_HX][AK_HY];
   __shared__ T sE[2][AK_TX + 1][AK_TY + 1];
   const int tid = threadIdx.x;
@@ -135,14 +142,23 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restri
     if (k >= zc0 && own) {  // node layer k complete
       const long long n = yy + (long long)g.NY * xx + plane * k;
       const uint8_t f = fix[n];
-      const double u0 = u[n], u1 = u[NN + n], u2 = u[2 * NN + n];
-      const TY r0 = (f & 1) ? TY(u0) : TY(a0[0]);
-      const TY r1 = (f & 2) ? TY(u1) : TY(a0[1]);
-      const TY r2 = (f & 4) ? TY(u2) : TY(a0[2]);
-      y[n] = r0;
-      y[NN + n] = r1;
-      y[2 * NN + n] = r2;
-      pq += u0 * double(r0) + u1 * double(r1) + u2 * double(r2);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double uc = u[c * NN + n];
+        const T ku = ((f >> c) & 1) ? T(uc) : a0[c];
+        if (EPI == EPI_KU) {
+          const TY v = TY(ku);
+          y[c * NN + n] = v;
+          pq += uc * double(v);
+        } else if (EPI == EPI_RESID) {
+          y[c * NN + n] = TY(T(rv[c * NN + n]) - ku);
+        } else {
+          const T rc = T(rv[c * NN + n]), dc = T(dv[c * NN + n]);
+          const TY v = dc != T(0) ? TY(T(uc) + T(omega) * (rc - ku) / dc) : TY(0);
+          y[c * NN + n] = v;
+          pq += double(v) * double(rv[c * NN + n]);
+        }
+      }
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -166,6 +182,124 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restri
   }
 }
 
+// Coarse-level K*u with the stored Galerkin cell matrices (Solving_KbyU_MatrixFree
+// for iLevel > 1, 727-744), same z-march as above: the thread of node column
+// (y, x) reads, for each of the <= 4 cells around the column in cell layer k,
+// the 6 rows of the cell's 24x24 matrix that belong to its two nodes (upper
+// triangle, entry-major: a wave reads one entry of consecutive cells), so
+// every stored entry is read about once.  Coarse levels carry no fixed DOFs
+// (the boundary conditions are inside the Galerkin matrices) and vectors are
+// zero on void nodes.  FP32 throughout (the 'Standard' V-cycle smoother).
+template <int CY, int CX>
+__device__ __forceinline__ void ks_cell_pair(const float* __restrict__ Ks, long long NN, long long cell,
+                                             const float (&n0)[27], const float (&n1)[27], float (&a0)[3],
+                                             float (&a1)[3]) {
+  constexpr int dy = 1 - CY, dx = 1 - CX;
+  constexpr int jb = loc_index(dy, dx, 0), jt = loc_index(dy, dx, 1);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int ky = CY + kLocDy[j], kx = CX + kLocDx[j];
+    const int nb = ky + 3 * kx;
+    const float* src = kLocDz[j] ? n1 : n0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = src[nb * 3 + c];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        a0[r] += Ks[(long long)tri_index(3 * jb + r, 3 * j + c) * NN + cell] * v;
+        a1[r] += Ks[(long long)tri_index(3 * jt + r, 3 * j + c) * NN + cell] * v;
+      }
+    }
+  }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(AK_T) void k_apply_Ks_tiled(Geo g, const float* __restrict__ Ks,
+                                                         const float* __restrict__ u, float* __restrict__ y,
+                                                         const float* __restrict__ rv, const float* __restrict__ dv,
+                                                         double omega, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  __shared__ float sU[2][3][AK_HX][AK_HY];
+  const int tid = threadIdx.x;
+  const int ty = tid % AK_TY, tx = tid / AK_TY;
+  const int y0 = blockIdx.x * AK_TY, x0 = blockIdx.y * AK_TX;
+  const int zc0 = blockIdx.z * AK_ZC;
+  const int zc1 = min(g.NZ, zc0 + AK_ZC);
+  const int yy = y0 + ty, xx = x0 + tx;
+  const bool own = yy < g.NY && xx < g.NX;
+  const long long NN = g.NN;
+  const long long plane = (long long)g.NY * g.NX;
+  auto stage = [&](int z, int b) {
+    for (int t = tid; t < AK_HX * AK_HY; t += AK_T) {
+      const int hy = t % AK_HY, hx = t / AK_HY;
+      const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+      float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+      if (z >= 0 && z < g.NZ && gy >= 0 && gx >= 0 && gy < g.NY && gx < g.NX) {
+        const long long n = gy + (long long)g.NY * gx + plane * z;
+        v0 = u[n];
+        v1 = u[NN + n];
+        v2 = u[2 * NN + n];
+      }
+      sU[b][0][hx][hy] = v0;
+      sU[b][1][hx][hy] = v1;
+      sU[b][2][hx][hy] = v2;
+    }
+  };
+  auto read_nb = [&](int b, float (&nbv)[27]) {
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nbv[(ky + 3 * kx) * 3 + c] = sU[b][c][tx + kx][ty + ky];
+  };
+  float n0[27], n1[27];
+  float a0[3] = {0.f, 0.f, 0.f}, a1[3] = {0.f, 0.f, 0.f};
+  const int kBeg = zc0 - 1;
+  stage(kBeg, kBeg & 1);
+  __syncthreads();
+  read_nb(kBeg & 1, n0);
+  stage(kBeg + 1, (kBeg + 1) & 1);
+  // the 4 cells around the column: (yy-1+CY, xx-1+CX), present if inside the cell grid
+  const bool c00 = own && yy - 1 >= 0 && xx - 1 >= 0 && yy - 1 < g.ny && xx - 1 < g.nx;
+  const bool c10 = own && yy >= 0 && xx - 1 >= 0 && yy < g.ny && xx - 1 < g.nx;
+  const bool c01 = own && yy - 1 >= 0 && xx >= 0 && yy - 1 < g.ny && xx < g.nx;
+  const bool c11 = own && yy < g.ny && xx < g.nx;
+  for (int k = kBeg; k < zc1; ++k) {
+    __syncthreads();
+    const int bk1 = (k + 1) & 1;
+    read_nb(bk1, n1);
+    __syncthreads();
+    if (k + 2 <= zc1) stage(k + 2, k & 1);
+    if (k >= 0 && k < g.nz) {
+      const long long base = (long long)yy + (long long)g.NY * xx + plane * k;  // cell (yy, xx, k)
+      if (c00) ks_cell_pair<0, 0>(Ks, NN, base - 1 - g.NY, n0, n1, a0, a1);
+      if (c10) ks_cell_pair<1, 0>(Ks, NN, base - g.NY, n0, n1, a0, a1);
+      if (c01) ks_cell_pair<0, 1>(Ks, NN, base - 1, n0, n1, a0, a1);
+      if (c11) ks_cell_pair<1, 1>(Ks, NN, base, n0, n1, a0, a1);
+    }
+    if (k >= zc0 && own) {
+      const long long n = yy + (long long)g.NY * xx + plane * k;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (EPI == EPI_RESID) {
+          y[c * NN + n] = rv[c * NN + n] - a0[c];
+        } else {
+          const float dc = dv[c * NN + n];
+          y[c * NN + n] = dc != 0.f ? u[c * NN + n] + float(omega) * (rv[c * NN + n] - a0[c]) / dc : 0.f;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      a0[r] = a1[r];
+      a1[r] = 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < 27; ++i) n0[i] = n1[i];
+  }
+}
+
 dim3 applyk_grid(const Geo& g) {
   return dim3((g.NY + AK_TY - 1) / AK_TY, (g.NX + AK_TX - 1) / AK_TX, (g.NZ + AK_ZC - 1) / AK_ZC);
 }
@@ -176,15 +310,36 @@ int applyk_tiled_blocks(const Geo& g) {
 }
 
 void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                          const double* u, double* y, double* partials, const int* fl, int gate) {
-  hipLaunchKernelGGL((k_apply_K_tiled<double, double, double, double>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E,
-                     fix, u, y, partials, fl, partials ? 1 : 0, gate);
+                          const double* u, double* y, double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<double, double, double, double, EPI_KU, double, double>), applyk_grid(g),
+                     dim3(AK_T), 0, s, g, Ke, E, fix, u, y, partials, fl, partials ? 1 : 0, (const double*)nullptr,
+                     (const double*)nullptr, 0.0);
 }
 
-void launch_apply_K_tiled_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                                const double* u, float* y, double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_tiled<float, float, double, float>), applyk_grid(g), dim3(AK_T), 0, s, g, Ke, E,
-                     fix, u, y, partials, fl, partials ? 1 : 0, -1);
+void launch_fine_resid32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                         const float* x, const double* r, float* d, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<float, float, float, float, EPI_RESID, double, float>), applyk_grid(g),
+                     dim3(AK_T), 0, s, g, Ke, E, fix, x, d, (double*)nullptr, fl, 0, r, (const float*)nullptr, 0.0);
+}
+
+void launch_fine_smooth32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                          const float* x, const double* r, const float* diag, double omega, float* out,
+                          double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<float, float, float, float, EPI_SMOOTH, double, float>), applyk_grid(g),
+                     dim3(AK_T), 0, s, g, Ke, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag, omega);
+}
+
+
+void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
+                           const int* fl) {
+  hipLaunchKernelGGL(k_apply_Ks_tiled<EPI_RESID>, applyk_grid(g), dim3(AK_T), 0, s, g, Ks, x, d, r,
+                     (const float*)nullptr, 0.0, fl);
+}
+
+void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
+                            const float* diag, double omega, float* out, const int* fl) {
+  hipLaunchKernelGGL(k_apply_Ks_tiled<EPI_SMOOTH>, applyk_grid(g), dim3(AK_T), 0, s, g, Ks, x, out, r, diag, omega,
+                     fl);
 }
 
 }  // namespace t3x

@@ -164,10 +164,11 @@ struct t3x_problem {
   double* bvec = nullptr;
   double *F = nullptr, *U = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp3 = nullptr;
   double *E = nullptr, *x = nullptr, *xPhys = nullptr, *ce = nullptr, *dc = nullptr;
-  // mixed precision (precision == 1): FP32 Ke, E, K*p output and V-cycle output
-  float *Ke32 = nullptr, *E32 = nullptr, *z32 = nullptr, *q32 = nullptr;
+  // mixed precision (precision == 1): FP32 V-cycle output; the 'Standard'
+  // cycle adds FP32 Ke/E and the fine-level smoother vectors
+  float* z32 = nullptr;
+  float *Ke32 = nullptr, *E32 = nullptr, *x32 = nullptr, *d32 = nullptr;
   void* rtmp = nullptr;  // intermediate of the separable restriction (largest level pair)
-  int lastReliable = 0;  // reliable updates in the last mixed solve
   double *ca = nullptr, *cb = nullptr, *cc = nullptr, *cd = nullptr;  // cell scratch
   // filters (node scalars)
   double *fb = nullptr, *fx = nullptr, *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr;
@@ -203,10 +204,12 @@ struct t3x_problem {
          *mxmax = nullptr;
   std::vector<void*> allocs;
 
+  long long bytes = 0;  // device bytes allocated by this problem
   template <class T>
   T* alloc(size_t n) {
     T* p = dalloc<T>(n);
     allocs.push_back(p);
+    bytes += (long long)(std::max<size_t>(n, 1) * sizeof(T));
     return p;
   }
   hipEvent_t ev() {
@@ -256,43 +259,54 @@ struct t3x_problem {
   double* xTilde() { return ca; }   // LOCAL-mode state between iterations
   double* volMax() { return cb; }
   long long NN() const { return lv[0].NN; }
-  bool mixed() const { return P.precision != 0; }  // FP32 V-cycle
+  bool mixed() const { return P.precision == 1; }  // FP32 V-cycle
+  bool standard() const { return P.vcycle == 1; }  // 'Standard' V-cycle (FP32)
   void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
 };
 
 namespace {
 
 // --------------------------------------------------------- hierarchy set-up
-// Level count and padding (1651-1671), non-dyadic merge (1085-1096).
-void build_levels(t3x_problem& P, const uint8_t* vox) {
-  long long nv = 0;
-  const long long nvox = (long long)P.nely * P.nelx * P.nelz;
-  for (long long i = 0; i < nvox; ++i) nv += vox[i] ? 1 : 0;
+// Level count and padding (1651-1671), non-dyadic merge (1085-1096):
+// per level (ny, nx, nz, span) of the padded grid.
+struct LevelDims {
+  int ny, nx, nz, span;
+};
+std::vector<LevelDims> level_dims(int nely, int nelx, int nelz, long long nv, int coarsestControl) {
   int cnt = 0;
-  while (nv >= P.P.coarsestControl) {
+  while (nv >= coarsestControl) {
     ++cnt;
     nv = (long long)std::llround((double)nv / 8.0);
   }
   cnt = std::max(3, cnt);
   const int pw = 1 << cnt;
   auto padded = [&](int n) { return (n + pw - 1) / pw * pw; };
-  P.L = cnt;  // (cnt + 1) levels, then one merged by the span-4 first coarsening
-  P.lv.assign(P.L, Level());
-  Level& F = P.lv[0];
-  F.ny = padded(P.nely); F.nx = padded(P.nelx); F.nz = padded(P.nelz);
-  for (int l = 1; l < P.L; ++l) {
-    Level& C = P.lv[l];
-    const Level& Fn = P.lv[l - 1];
-    C.span = (l == 1) ? 4 : 2;
-    C.ny = Fn.ny / C.span; C.nx = Fn.nx / C.span; C.nz = Fn.nz / C.span;
+  // (cnt + 1) levels, then one merged by the span-4 first coarsening
+  std::vector<LevelDims> d(cnt);
+  d[0] = {padded(nely), padded(nelx), padded(nelz), 0};
+  for (int l = 1; l < cnt; ++l) {
+    const int s = (l == 1) ? 4 : 2;
+    d[l] = {d[l - 1].ny / s, d[l - 1].nx / s, d[l - 1].nz / s, s};
   }
+  return d;
+}
+
+void build_levels(t3x_problem& P, const uint8_t* vox) {
+  long long nv = 0;
+  const long long nvox = (long long)P.nely * P.nelx * P.nelz;
+  for (long long i = 0; i < nvox; ++i) nv += vox[i] ? 1 : 0;
+  const std::vector<LevelDims> dims = level_dims(P.nely, P.nelx, P.nelz, nv, P.P.coarsestControl);
+  P.L = (int)dims.size();
+  P.lv.assign(P.L, Level());
   for (int l = 0; l < P.L; ++l) {
     Level& V = P.lv[l];
+    V.ny = dims[l].ny; V.nx = dims[l].nx; V.nz = dims[l].nz; V.span = dims[l].span;
     V.NY = V.ny + 1; V.NX = V.nx + 1; V.NZ = V.nz + 1;
     V.NN = (long long)V.NY * V.NX * V.NZ;
     V.hEmask.assign(V.NN, 0);
     V.hNmask.assign(V.NN, 0);
   }
+  Level& F = P.lv[0];
   // finest cells
   for (int iz = 0; iz < P.nelz; ++iz)
     for (int ix = 0; ix < P.nelx; ++ix)
@@ -325,6 +339,61 @@ void build_levels(t3x_problem& P, const uint8_t* vox) {
     V.nNodes = 0;
     for (long long n = 0; n < V.NN; ++n) V.nNodes += V.hNmask[n];
   }
+}
+
+// Upper bound of the device bytes setup() and a whole run allocate: the same
+// per-level arrays, the coarsest band with the bandwidth of a full box in
+// its best axis order (a sub-domain's ordering is never wider), and 4 KiB of
+// allocation slack per array.
+long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t3x_params& prm, int mode) {
+  const std::vector<LevelDims> d = level_dims(nely, nelx, nelz, nv, prm.coarsestControl);
+  const int L = (int)d.size();
+  const bool mixed = prm.precision == 1;
+  auto NNof = [&](int l) { return (long long)(d[l].ny + 1) * (d[l].nx + 1) * (d[l].nz + 1); };
+  long long b = 0, nalloc = 0;
+  auto add = [&](long long bytes, int count = 1) { b += bytes * count; nalloc += count; };
+  long long tmp = 0;
+  for (int l = 0; l < L; ++l) {
+    const long long NN = NNof(l);
+    add(NN, 2);                                      // emask, nmask
+    if (l < L - 1) add(24 * NN);                     // diag
+    if (l >= 1) add(24 * NN, 2), add(8LL * kTri * NN);  // r, x, Ks
+    if (mixed) {
+      if (l < L - 1) add(12 * NN);                   // diagf
+      if (l >= 1) add(12 * NN, 2);                   // rf, xf
+      if (prm.vcycle == 1 && l >= 1 && l < L - 1) add(12 * NN), add(4LL * kTri * NN);  // tf, Ksf
+    }
+    if (l + 1 < L) tmp = std::max(tmp, 3LL * (d[l + 1].ny + 1) * (d[l].nx + 1) * (d[l].nz + 1));
+  }
+  add(8 * tmp);                                      // restriction intermediate
+  const long long NN = NNof(0);
+  add(NN);                                           // fix
+  add(24 * NN, 8);                                   // F, U, r, z, p, q, tmp3, bvec
+  add(8 * NN, 9);                                    // cell arrays
+  add(8 * NN, 6);                                    // filter vectors
+  add(8 * NN, 2);                                    // filter diagonals
+  if (mixed) add(12 * NN);                           // z32
+  if (prm.vcycle == 1) add(576 * 4), add(4 * NN), add(12 * NN, 2);  // Ke32, E32, x32, d32
+  if (mode == 1) add(8 * NN, 13);                    // MMA state
+  add(576 * 8);                                      // Ke
+  add(64LL * kTri * 8);                              // level-2 templates
+  add(4 * NNof(1) + NNof(1));                        // BC cell list + flags
+  add(8 * NN, 1);                                    // compact node + cell maps (2 x int32)
+  // coarsest band
+  int c[3] = {d[L - 1].ny + 1, d[L - 1].nx + 1, d[L - 1].nz + 1};
+  std::sort(c, c + 3);
+  const long long nC = 3LL * c[0] * c[1] * c[2];
+  const long long bw = 3LL * (1 + c[0] + (long long)c[0] * c[1]) + 2;
+  const long long SB = std::max<long long>(64, (bw + 1 + 63) / 64 * 64);
+  const long long np = (nC + SB - 1) / SB * SB;
+  add(8 * nC);                                       // coarsest map
+  add(4 * NNof(L - 1));                              // node positions
+  add(8 * np * (bw + 1));                            // band
+  add(8 * np * 64);                                  // tile inverses
+  add(8 * (np / SB) * SB * SB, 4);                   // Dinv, DT, G, H
+  add(8 * np, 2);                                    // solve vectors
+  add(8LL * std::max(kMaxRedBlocks * 8, kMaxPartials) + 4096 * 4);  // reductions, scalars, flags
+  return b + 4096 * nalloc;
 }
 
 int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, const uint8_t* fixState, int nLoad,
@@ -405,11 +474,15 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       V.x = P.alloc<double>(3 * V.NN);
       V.Ks = P.alloc<double>((size_t)kTri * V.NN);
     }
-    if (P.P.precision != 0) {
+    if (P.mixed()) {
       if (l < P.L - 1) V.diagf = P.alloc<float>(3 * V.NN);
       if (l >= 1) {
         V.rf = P.alloc<float>(3 * V.NN);
         V.xf = P.alloc<float>(3 * V.NN);
+      }
+      if (P.standard() && l >= 1 && l < P.L - 1) {
+        V.tf = P.alloc<float>(3 * V.NN);
+        V.Ksf = P.alloc<float>((size_t)kTri * V.NN);
       }
     }
   }
@@ -429,13 +502,14 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   const std::vector<double> Ke = gauss_ke(P.P.nu, 1.0);
   P.dKe = P.alloc<double>(576);
   T3X_HIP(hipMemcpy(P.dKe, Ke.data(), 576 * sizeof(double), hipMemcpyHostToDevice));
-  if (P.P.precision != 0) {
+  if (P.mixed()) P.z32 = P.alloc<float>(3 * NN);
+  if (P.standard()) {
     std::vector<float> Kf(Ke.begin(), Ke.end());
     P.Ke32 = P.alloc<float>(576);
     T3X_HIP(hipMemcpy(P.Ke32, Kf.data(), 576 * sizeof(float), hipMemcpyHostToDevice));
     P.E32 = P.alloc<float>(NN);
-    P.z32 = P.alloc<float>(3 * NN);
-    P.q32 = P.alloc<float>(3 * NN);
+    P.x32 = P.alloc<float>(3 * NN);
+    P.d32 = P.alloc<float>(3 * NN);
   }
   const std::vector<double> T2 = l2_templates(Ke, P.lv[1].span);
   P.dT2 = P.alloc<double>(T2.size());
@@ -527,7 +601,7 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   T3X_HIP(hipMemcpy(P.dNodeMap, P.hNodeMap.data(), P.nNodes * sizeof(int), hipMemcpyHostToDevice));
   T3X_HIP(hipMemcpy(P.dCellMap, P.hCellMap.data(), P.nCells * sizeof(int), hipMemcpyHostToDevice));
   // ---- reductions and scalars
-  P.ws.partials = P.alloc<double>((size_t)kMaxRedBlocks * 8);
+  P.ws.partials = P.alloc<double>((size_t)std::max(kMaxRedBlocks * 8, kMaxPartials));
   P.ws.counter = P.alloc<unsigned int>(4);
   P.sc = P.alloc<double>(64);
   P.fl = P.alloc<int>(16);
@@ -557,9 +631,11 @@ int assemble(t3x_problem& P) {
     if (l < P.L - 1) launch_diag_coarse(s, gc, C.Ks, C.emask, C.nmask, C.diag);
   }
   launch_diag_fine(s, g0, P.dKe, P.E, P.fix, P.lv[0].nmask, P.lv[0].diag);
-  if (P.P.precision != 0) {
+  if (P.mixed())
     for (int l = 0; l < P.L - 1; ++l) launch_convert(s, P.lv[l].diagf, P.lv[l].diag, 3 * P.lv[l].NN);
+  if (P.standard()) {
     launch_convert(s, P.E32, P.E, P.NN());
+    for (int l = 1; l < P.L - 1; ++l) launch_convert(s, P.lv[l].Ksf, P.lv[l].Ks, (long long)kTri * P.lv[l].NN);
   }
   const Level& C = P.lv[P.L - 1];
   launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, P.band, (long long)P.cs.band_len());
@@ -626,26 +702,62 @@ void vcycle_mixed(t3x_problem& P, const double* r, float* z, double* sc, const i
   P.tend(TG_VCYCLE, ev);
 }
 
+// 'Standard' V-cycle (759-777) in FP32: x = w r/d; d = r - K x; restrict d
+// down to the coarsest FP64 solve; then x += P x_coarse; x += w (r - K x)/d
+// on the way up.  Fine-level K*u with E*Ke and the BC rows, coarse levels
+// with the FP32 copies of the Galerkin matrices; z.r fused into the last
+// smoothing step.
+void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, const int* fl, bool fuse) {
+  hipStream_t s = P.stream;
+  const int L = P.L;
+  const double om = P.P.omega;
+  float* tmp = (float*)P.rtmp;
+  const Geo g0 = geo(P.lv[0]);
+  hipEvent_t ev[2], ec[2];
+  P.tbegin(TG_VCYCLE, ev);
+  launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl);                      // 757
+  launch_fine_resid32(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.d32, fl);                   // 761
+  launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const float*)P.d32, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
+  for (int l = 1; l < L - 1; ++l) {
+    Level& V = P.lv[l];
+    launch_jacobi0(s, (const float*)V.rf, V.diagf, V.xf, om, 3 * V.NN, fl);
+    launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, fl);
+    launch_restrict(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const float*)V.tf, tmp, P.lv[l + 1].nmask,
+                    P.lv[l + 1].rf, fl);
+  }
+  Level& C = P.lv[L - 1];
+  P.tbegin(TG_COARSE, ec);
+  P.cs.solve(s, P.dCMap, C.rf, C.xf, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);      // 767
+  P.tend(TG_COARSE, ec);
+  for (int l = L - 2; l >= 1; --l) {
+    Level& V = P.lv[l];
+    launch_prolong_add(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, P.lv[l + 1].xf, V.xf, V.nmask, fl);  // 770
+    launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, fl);          // 775
+    std::swap(V.xf, V.tf);
+  }
+  launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, P.lv[1].xf, P.x32, P.lv[0].nmask, fl);
+  launch_fine_smooth32(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z, fuse ? P.ws.partials : nullptr,
+                       fl);
+  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_tiled_blocks(g0), P.ws, sc, (int*)fl, nullptr);
+  P.tend(TG_VCYCLE, ev);
+}
+
 // ------------------------------------------------------------- PCG
 // Solving_PCG (600-639) with device-side scalars and convergence flag.
-// Reliable-update factor of the mixed form: a true FP64 residual replaces the
-// recursive one whenever the latter has dropped by this factor, and before
-// the solve may stop.
-constexpr double kReliableDelta = 0.1;
-
 int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxIT, double* resOut, bool* warned) {
   hipStream_t s = P.stream;
   const long long n3 = 3 * P.NN();
   double* sc = P.sc;
   int* fl = P.fl;
   const Geo g0 = geo(P.lv[0]);
-  const int prec = P.P.precision;
-  launch_pcg_init(s, sc, fl, tol, prec == 1 ? kReliableDelta : 0.0);
+  const bool mx = P.mixed();
+  launch_pcg_init(s, sc, fl, tol);
   launch_norm2(s, b, n3, P.ws, &sc[SC_NORMB]);                                              // 607
-  launch_apply_K(s, g0, P.dKe, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);               // 609 (FP64)
+  launch_apply_K(s, g0, P.dKe, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);               // 609
   launch_sub(s, P.r, b, P.q, n3);
-  if (prec) {
-    vcycle_mixed(P, P.r, P.z32, sc, nullptr, true);                                         // 610, 612
+  if (mx) {
+    if (P.standard()) vcycle_standard(P, P.r, P.z32, sc, nullptr, true);                   // 610, 612
+    else vcycle_mixed(P, P.r, P.z32, sc, nullptr, true);
     launch_convert(s, P.p, P.z32, n3);                                                      // 611
   } else {
     vcycle(P, P.r, P.z, sc, nullptr, true);
@@ -657,19 +769,12 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
   for (int it = 1; it <= maxIT; ++it) {
     hipEvent_t ek[2];
     P.tbegin(TG_APPLYK, ek);
-    if (prec == 1)
-      launch_apply_K_mixed(s, g0, P.Ke32, P.E32, P.fix, P.p, P.q32, P.ws, sc, fl, true);    // 614-615
-    else
-      launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);
+    launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                 // 614-615
     P.tend(TG_APPLYK, ek);
-    if (prec == 1) {
-      launch_pcg_update(s, P.r, P.q32, n3, P.ws, sc, fl);                                   // 617-622
-      launch_reliable_update(s, g0, P.dKe, P.E, P.fix, b, y, P.p, P.q, P.r, n3, P.ws, sc, fl);
-    } else {
-      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);
-    }
-    if (prec) {
-      vcycle_mixed(P, P.r, P.z32, sc, fl, true);                                            // 628-629
+    launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                               // 617-622
+    if (mx) {
+      if (P.standard()) vcycle_standard(P, P.r, P.z32, sc, fl, true);                      // 628-629
+      else vcycle_mixed(P, P.r, P.z32, sc, fl, true);
       launch_pcg_pupdate(s, y, P.p, P.z32, n3, sc, fl);                                     // 616, 630
     } else {
       vcycle(P, P.r, P.z, sc, fl, true);
@@ -684,10 +789,8 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
   }
   launch_pcg_finish(s, y, P.p, n3, sc, fl);
   T3X_HIP(hipMemcpyAsync(P.hbuf, sc, SC_COUNT_PCG * sizeof(double), hipMemcpyDeviceToHost, s));
-  T3X_HIP(hipMemcpyAsync(P.hbuf + 32, fl, FL_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
   P.sync();
   if (resOut) *resOut = P.hbuf[SC_RES];
-  P.lastReliable = ((const int*)(P.hbuf + 32))[FL_NREL];
   if (warned) *warned = !done;  // 634-637
   return its;
 }
@@ -1077,6 +1180,7 @@ void t3x_default_params(t3x_params* p) {
   p->precision = 0;
   p->checkEvery = 4;
   p->timing = 0;
+  p->vcycle = 0;  // 'Adapted' (34)
 }
 
 int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
@@ -1092,8 +1196,9 @@ int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, co
     *out = P.release();
     return T3X_ERR_INPUT;
   }
-  if (P->P.precision < 0 || P->P.precision > 2) {
-    P->err = "params.precision must be 0 (FP64), 1 (mixed) or 2 (FP32 V-cycle only)";
+  if ((P->P.precision != 0 && P->P.precision != 1) || (P->P.vcycle != 0 && P->P.vcycle != 1) ||
+      (P->P.vcycle == 1 && P->P.precision != 1)) {
+    P->err = "params: precision must be 0 (FP64) or 1 (mixed); vcycle 1 ('Standard') needs precision 1";
     *out = P.release();
     return T3X_ERR_INPUT;
   }
@@ -1161,6 +1266,60 @@ int t3x_kernel_time(t3x_problem* h, int group, long long* count, double* ms) {
   });
 }
 
+int t3x_footprint(const uint8_t* voxels, int nely, int nelx, int nelz, const t3x_params* params, int mode,
+                  long long* bytes) {
+  if (!bytes || nelx < 3 || nely < 3 || nelz < 3) return T3X_ERR_INPUT;
+  t3x_params P;
+  if (params) P = *params; else t3x_default_params(&P);
+  long long nv = (long long)nely * nelx * nelz;
+  if (voxels) {
+    nv = 0;
+    for (long long i = 0; i < (long long)nely * nelx * nelz; ++i) nv += voxels[i] ? 1 : 0;
+  }
+  *bytes = footprint_estimate(nely, nelx, nelz, nv, P, mode);
+  return T3X_OK;
+}
+
+int t3x_stream_copy(long long bytes, int reps, double* msPerCopy) {
+  if (bytes < 16 || reps < 1 || !msPerCopy) return T3X_ERR_INPUT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return T3X_ERR_NO_DEVICE;
+  const long long n = bytes / 16 * 2;  // doubles, a whole number of 16-byte pairs
+  double *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = T3X_OK;
+  try {
+    a = dalloc<double>(n);
+    b = dalloc<double>(n);
+    T3X_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    T3X_HIP(hipEventCreate(&e0));
+    T3X_HIP(hipEventCreate(&e1));
+    launch_stream_copy(s, b, a, n);  // warm-up
+    T3X_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) launch_stream_copy(s, (i & 1) ? a : b, (i & 1) ? b : a, n);
+    T3X_HIP(hipEventRecord(e1, s));
+    T3X_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    T3X_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *msPerCopy = ms / reps;
+  } catch (const HipError&) {
+    rc = T3X_ERR_HIP;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
+int t3x_memory(const t3x_problem* h, long long* bytes) {
+  if (!h || !bytes) return T3X_ERR_INPUT;
+  *bytes = h->bytes;
+  return T3X_OK;
+}
+
 int t3x_kernel_time_reset(t3x_problem* h) {
   return guarded(h, [&] {
     for (int g = 0; g < TG_COUNT; ++g) {
@@ -1196,7 +1355,8 @@ int t3x_vcycle(t3x_problem* h, const double* r, double* x) {
     if (!h->assembled) { h->err = "t3x_vcycle before t3x_assemble"; return (int)T3X_ERR_INPUT; }
     upload_nodal(*h, r, h->r);
     if (h->mixed()) {
-      vcycle_mixed(*h, h->r, h->z32, h->sc, nullptr, false);
+      if (h->standard()) vcycle_standard(*h, h->r, h->z32, h->sc, nullptr, false);
+      else vcycle_mixed(*h, h->r, h->z32, h->sc, nullptr, false);
       launch_convert(h->stream, h->z, h->z32, 3 * h->NN());
     } else {
       vcycle(*h, h->r, h->z, h->sc, nullptr, false);

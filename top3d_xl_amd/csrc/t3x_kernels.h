@@ -8,14 +8,9 @@ namespace t3x {
 // Scalar slots in device memory (double sc[] / int fl[]) for the solvers.
 enum ScalarSlot {
   SC_NORMB = 0, SC_ZR, SC_PQ, SC_RR, SC_ZRN, SC_RES, SC_LAMBDA, SC_BETA,
-  SC_TOL,
-  SC_DELTA,  // reliable-update factor (0: off)
-  SC_RLAST,  // relative residual at the last reliable update (< 0: not yet set)
-  SC_COUNT_PCG  // one PCG block = SC_COUNT_PCG doubles (<= 16)
+  SC_TOL, SC_COUNT_PCG  // one PCG block = SC_COUNT_PCG doubles (<= 16)
 };
-// FL_REL: a reliable update (true FP64 residual) is due this iteration;
-// FL_YCUR: y already holds this iteration's lambda*p.
-enum FlagSlot { FL_DONE = 0, FL_ITS, FL_ITER, FL_REL, FL_YCUR, FL_NREL, FL_COUNT };  // <= 8
+enum FlagSlot { FL_DONE = 0, FL_ITS, FL_ITER, FL_COUNT };  // <= 8
 
 // Geometry passed by value to kernels.
 struct Geo {
@@ -40,12 +35,7 @@ void launch_fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int
 // tiled K*u (t3x_applyk.hip); partials: one per work-group (nullptr: none)
 int applyk_tiled_blocks(const Geo& g);
 void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                          const double* u, double* y, double* partials, const int* fl, int gate = -1);
-// mixed precision: FP64 input vector, FP32 Ke/E/arithmetic/output, FP64 u.y partials
-void launch_apply_K_tiled_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                                const double* u, float* y, double* partials, const int* fl);
-void launch_apply_K_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                          const double* u, float* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
+                          const double* u, double* y, double* partials, const int* fl);
 void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                            const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
@@ -83,6 +73,25 @@ void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span
 void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const float* xc,
                            const double* rf, const float* diag, const uint8_t* nmaskF, float* out,
                            const RedWs* ws, double* sc, const int* fl);
+// 'Standard' V-cycle pieces (FP32): x += P xc; x = omega r / d
+void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* xc, float* x,
+                        const uint8_t* nmaskF, const int* fl);
+void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
+                    const int* fl);
+void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
+                    const int* fl);
+// FP32 K*u smoothers (t3x_applyk.hip): fine level with E*Ke and BC rows, coarse
+// levels with the stored Galerkin matrices (FP32 copy).  resid: d = r - Kx;
+// smooth: out = x + omega (r - Kx) / diag (fine: fused out.r partials).
+void launch_fine_resid32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                         const float* x, const double* r, float* d, const int* fl);
+void launch_fine_smooth32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                          const float* x, const double* r, const float* diag, double omega, float* out,
+                          double* partials, const int* fl);
+void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
+                           const int* fl);
+void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
+                            const float* diag, double omega, float* out, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const float* r, double* v, const int* fl);
 void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long NN3,
@@ -107,18 +116,13 @@ void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, con
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
                         const int* fl);
 void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl);
-// mixed precision: FP32 q = K p and z = M r, FP64 y, r, p
-void launch_pcg_update(hipStream_t s, double* r, const float* q, long long n, const RedWs& ws, double* sc, int* fl);
+// mixed precision: FP32 z = M r, FP64 y, r, p, q
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const float* z, long long n, const double* sc,
                         const int* fl);
+void launch_stream_copy(hipStream_t s, double* dst, const double* src, long long nDoubles);  // nDoubles even
 void launch_convert(hipStream_t s, float* dst, const double* src, long long n);
 void launch_convert(hipStream_t s, double* dst, const float* src, long long n);
-void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol, double delta = 0.0);
-// Reliable update of the mixed PCG, every kernel a no-op unless fl[FL_REL]:
-// y += lambda p; q = K y (FP64); r = b - q; the true residual alone may end the solve.
-void launch_reliable_update(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                            const double* b, double* y, const double* p, double* q, double* r, long long n,
-                            const RedWs& ws, double* sc, int* fl);
+void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol);
 void launch_scale_diag(hipStream_t s, double* z, const double* dinv, const double* r, long long n,
                        const RedWs* ws, double* sc, const int* fl);
 void launch_gather_compact(hipStream_t s, const int* map, int n, int ncomp, long long NN, const double* dense,

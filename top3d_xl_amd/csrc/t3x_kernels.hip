@@ -49,7 +49,6 @@ __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* _
     case FIN_PQ:  // 615
       sc[SC_PQ] = t;
       sc[SC_LAMBDA] = sc[SC_ZR] / t;
-      fl[FL_YCUR] = 0;
       break;
     case FIN_ZR_INIT:  // 612
       sc[SC_ZR] = t;
@@ -65,16 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* _
       sc[SC_RES] = res;
       const int it = fl[FL_ITER] + 1;
       fl[FL_ITER] = it;
-      const double delta = sc[SC_DELTA];
-      if (delta > 0.0) {
-        // mixed precision: the recursive residual only schedules a true
-        // FP64 residual (launch_reliable_update), which alone may end the solve
-        if (sc[SC_RLAST] < 0.0) sc[SC_RLAST] = res;
-        if (res < sc[SC_TOL] || res < delta * sc[SC_RLAST]) {
-          fl[FL_REL] = 1;
-          fl[FL_YCUR] = 1;
-        }
-      } else if (res < sc[SC_TOL]) {
+      if (res < sc[SC_TOL]) {
         fl[FL_DONE] = 1;
         fl[FL_ITS] = it;
       }
@@ -170,11 +160,6 @@ void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const 
 void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                     const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
   launch_apply_K_tiled(s, g, Ke, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
-  if (pcgStep) fin1(s, FIN_PQ, applyk_tiled_blocks(g), ws, sc, (int*)fl, nullptr);
-}
-void launch_apply_K_mixed(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                          const double* u, float* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
-  launch_apply_K_tiled_mixed(s, g, Ke, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
   if (pcgStep) fin1(s, FIN_PQ, applyk_tiled_blocks(g), ws, sc, (int*)fl, nullptr);
 }
 
@@ -754,7 +739,7 @@ void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, lo
 // y += lambda p is applied here (no-op if the solve hit maxIT).
 __global__ void k_pcg_finish(double* __restrict__ y, const double* __restrict__ p, long long n, const double* sc,
                              const int* fl) {
-  if (!fl[FL_DONE] || fl[FL_YCUR]) return;
+  if (!fl[FL_DONE]) return;
   const double lambda = sc[SC_LAMBDA];
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
     y[i] = y[i] + lambda * p[i];
@@ -763,28 +748,11 @@ void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, c
   hipLaunchKernelGGL(k_pcg_finish, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, n, sc, fl);
 }
 
-// Mixed precision (FP32 q and z, FP64 r, p, y and scalars): same steps.
-__global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(double* __restrict__ r, const float* __restrict__ q,
-                                                             long long n, RedWs ws, double* sc, int* fl) {
-  if (fl[FL_DONE]) return;
-  const double lambda = sc[SC_LAMBDA];
-  double rr = 0.0;
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    const double ri = r[i] - lambda * (double)q[i];
-    r[i] = ri;
-    rr += ri * ri;
-  }
-  double v[1] = {rr};
-  write_partials<1>(v, ws.partials);
-}
-void launch_pcg_update(hipStream_t s, double* r, const float* q, long long n, const RedWs& ws, double* sc, int* fl) {
-  hipLaunchKernelGGL(k_pcg_update_mixed, dim3(red_blocks(n)), dim3(kBlock), 0, s, r, q, n, ws, sc, fl);
-  fin1(s, FIN_RR, red_blocks(n), ws, sc, fl, nullptr);
-}
+// Mixed precision: the V-cycle output z is FP32; y, r, p and scalars FP64.
 __global__ void k_pcg_pupdate_mixed(double* __restrict__ y, double* __restrict__ p, const float* __restrict__ z,
                                     long long n, const double* sc, const int* fl) {
   if (fl[FL_DONE]) return;
-  const double beta = sc[SC_BETA], lambda = fl[FL_YCUR] ? 0.0 : sc[SC_LAMBDA];
+  const double beta = sc[SC_BETA], lambda = sc[SC_LAMBDA];
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
     const double pi = p[i];
     y[i] = y[i] + lambda * pi;
@@ -807,60 +775,12 @@ void launch_convert(hipStream_t s, double* dst, const float* src, long long n) {
   hipLaunchKernelGGL((k_convert<double, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
 }
 
-__global__ void k_pcg_init(double* sc, int* fl, double tol, double delta) {
+__global__ void k_pcg_init(double* sc, int* fl, double tol) {
   sc[SC_TOL] = tol;
-  sc[SC_DELTA] = delta;
-  sc[SC_RLAST] = -1.0;
   for (int i = 0; i < FL_COUNT; ++i) fl[i] = 0;
 }
-void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol, double delta) {
-  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1), 0, s, sc, fl, tol, delta);
-}
-
-__global__ void k_rel_axpy(double* __restrict__ y, const double* __restrict__ p, long long n, const double* sc,
-                           const int* fl) {
-  if (fl[FL_DONE] || !fl[FL_REL]) return;
-  const double lambda = sc[SC_LAMBDA];
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-    y[i] = y[i] + lambda * p[i];
-}
-__global__ __launch_bounds__(kBlock) void k_rel_residual(double* __restrict__ r, const double* __restrict__ b,
-                                                         const double* __restrict__ q, long long n, RedWs ws,
-                                                         const int* fl) {
-  if (fl[FL_DONE] || !fl[FL_REL]) return;
-  double rr = 0.0;
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    const double ri = b[i] - q[i];
-    r[i] = ri;
-    rr += ri * ri;
-  }
-  double v[1] = {rr};
-  write_partials<1>(v, ws.partials);
-}
-__global__ __launch_bounds__(kBlock) void k_fin_rrt(int nb, const double* __restrict__ partials, double* sc, int* fl) {
-  if (fl[FL_DONE] || !fl[FL_REL]) return;
-  double v[1];
-  fold_partials<1>(partials, nb, v);
-  if (threadIdx.x != 0) return;
-  const double t = v[0];
-  const double res = sqrt(t) / sc[SC_NORMB];
-  sc[SC_RR] = t;
-  sc[SC_RES] = res;
-  sc[SC_RLAST] = res;
-  fl[FL_REL] = 0;
-  fl[FL_NREL] += 1;
-  if (res < sc[SC_TOL]) {
-    fl[FL_DONE] = 1;
-    fl[FL_ITS] = fl[FL_ITER];
-  }
-}
-void launch_reliable_update(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                            const double* b, double* y, const double* p, double* q, double* r, long long n,
-                            const RedWs& ws, double* sc, int* fl) {
-  hipLaunchKernelGGL(k_rel_axpy, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, n, sc, fl);
-  launch_apply_K_tiled(s, g, Ke, E, fix, y, q, nullptr, fl, FL_REL);
-  hipLaunchKernelGGL(k_rel_residual, dim3(red_blocks(n)), dim3(kBlock), 0, s, r, b, q, n, ws, fl);
-  hipLaunchKernelGGL(k_fin_rrt, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, sc, fl);
+void launch_pcg_init(hipStream_t s, double* sc, int* fl, double tol) {
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(1), 0, s, sc, fl, tol);
 }
 
 // z = dinv .* r (889) with the fused z.r reduction.
@@ -1457,6 +1377,20 @@ void launch_mma_step(hipStream_t s, long long n, const uint8_t* emask, double* x
   hipLaunchKernelGGL(k_mma_stepB, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, x, low, upp, alpha, beta, p0,
                      q0, pij, qij, ws, mm);
   hipLaunchKernelGGL(k_mma_stepB_fin, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, mm);
+}
+
+// =========================================================================
+// Streaming copy of known size: the practical HBM-bandwidth reference and
+// the calibration kernel for hardware byte counters (16-byte accesses).
+// =========================================================================
+__global__ __launch_bounds__(kBlock) void k_stream_copy(double2* __restrict__ dst, const double2* __restrict__ src,
+                                                        long long n) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+void launch_stream_copy(hipStream_t s, double* dst, const double* src, long long nDoubles) {
+  hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(kBlock), 0, s, (double2*)dst, (const double2*)src,
+                     nDoubles / 2);
 }
 
 }  // namespace t3x

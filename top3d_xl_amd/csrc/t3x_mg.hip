@@ -11,8 +11,9 @@
 // pass (-> coarse), each thread owning one output, a wave reading
 // consecutive y.  The y pass reads every fine value once from HBM; the
 // intermediate is 1/S of the fine vector.  Prolongation + Jacobi runs one
-// thread per (y, x) column walking z, so the coarse-column weights are
-// computed once per thread and no index division happens in the z loop.
+// thread per (y, x) column over a short run of z planes, so the coarse-column
+// weights are computed once per thread and the grid still has tens of
+// thousands of blocks.
 //
 // Templates over the vector types give the FP64 path (double everywhere) and
 // the mixed path (FP64 fine residual, FP32 coarse levels and smoother
@@ -103,9 +104,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict_xz(Geo gf, Geo gc, const TM
 }
 
 // out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere (758,
-// 770, 773); optional fused z.r partials (629).
-template <int S, typename TX, typename TR, typename TD, typename TO>
-__global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega, const TX* __restrict__ xc,
+// 770, 773); optional fused z.r partials (629).  Block (bx, bz) covers 256
+// (y, x) columns and the node planes [bz*zc, bz*zc + zc); every load is in
+// bounds (zero weights read a clamped neighbour), so the loads of a plane
+// issue together and the mask only selects the result.
+// ADD: out += P xc instead (the 'Standard' cycle's coarse correction, 770); rf
+// and diag are then unused.
+template <int S, typename TX, typename TR, typename TD, typename TO, bool ADD = false>
+__global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega, int zc, const TX* __restrict__ xc,
                                                     const TR* __restrict__ rf, const TD* __restrict__ diag,
                                                     const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
                                                     double* __restrict__ partials, const int* fl) {
@@ -113,46 +119,47 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
   const int plane = gf.NY * gf.NX;
   const long long NN = gf.NN, NC = gc.NN;
   const long long planeC = (long long)gc.NY * gc.NX;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
   double zr = 0.0;
-  for (int idx = blockIdx.x * kBlock + threadIdx.x; idx < plane; idx += gridDim.x * kBlock) {
+  if (idx < plane) {
     const int x = idx / gf.NY, y = idx - x * gf.NY;
     const int Jy = y / S, Jx = x / S;
     const int ty = y - S * Jy, tx = x - S * Jx;
     const double wy[2] = {(double)(S - ty) * (1.0 / S), (double)ty * (1.0 / S)};
     const double wx[2] = {(double)(S - tx) * (1.0 / S), (double)tx * (1.0 / S)};
-    // clamped neighbours: a zero weight never reads past the last coarse node
     const int Jy1 = min(Jy + 1, gc.NY - 1), Jx1 = min(Jx + 1, gc.NX - 1);
     const long long cyx[4] = {Jy + (long long)gc.NY * Jx, Jy1 + (long long)gc.NY * Jx, Jy + (long long)gc.NY * Jx1,
                               Jy1 + (long long)gc.NY * Jx1};
-    for (int z = blockIdx.y; z < gf.NZ; z += gridDim.y) {
+    const int z0 = blockIdx.y * zc, z1 = min(gf.NZ, z0 + zc);
+    for (int z = z0; z < z1; ++z) {
       const long long n = idx + (long long)plane * z;
-      double o[3] = {0.0, 0.0, 0.0};
-      if (nmaskF[n]) {
-        const int Jz = z / S, tz = z - S * Jz;
-        const int Jz1 = min(Jz + 1, gc.NZ - 1);
-        const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
-        const long long zo[2] = {planeC * Jz, planeC * Jz1};
-        double p[3] = {0.0, 0.0, 0.0};
+      const bool solid = nmaskF[n] != 0;
+      const int Jz = z / S, tz = z - S * Jz;
+      const int Jz1 = min(Jz + 1, gc.NZ - 1);
+      const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
+      const long long zo[2] = {planeC * Jz, planeC * Jz1};
+      double p[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
-          const double w = wy[a] * wx[b] * wz[d];
-          const long long J = cyx[a + 2 * b] + zo[d];
-          p[0] += w * (double)xc[J];
-          p[1] += w * (double)xc[NC + J];
-          p[2] += w * (double)xc[2 * NC + J];
-        }
+      for (int c = 0; c < 8; ++c) {
+        const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
+        const double w = wy[a] * wx[b] * wz[d];
+        const long long J = cyx[a + 2 * b] + zo[d];
+        p[0] += w * (double)xc[J];
+        p[1] += w * (double)xc[NC + J];
+        p[2] += w * (double)xc[2 * NC + J];
+      }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const double r = (double)rf[k * NN + n], dd = (double)diag[k * NN + n];
-          const TO v = (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd);
-          o[k] = (double)v;
-          zr += o[k] * r;
+      for (int k = 0; k < 3; ++k) {
+        if (ADD) {
+          out[k * NN + n] = solid ? (TO)((double)out[k * NN + n] + p[k]) : TO(0);
+        } else {
+          const double r = (double)rf[k * NN + n];
+          const double dd = solid ? (double)diag[k * NN + n] : 1.0;
+          const TO v = solid ? (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd) : TO(0);
+          zr += (double)v * r;
+          out[k * NN + n] = v;
         }
       }
-      out[n] = (TO)o[0];
-      out[NN + n] = (TO)o[1];
-      out[2 * NN + n] = (TO)o[2];
     }
   }
   if (partials) {
@@ -177,15 +184,13 @@ void restrict_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const 
   }
 }
 
-dim3 prolong_grid(const Geo& gf, bool reduce) {
-  const int plane = gf.NY * gf.NX;
-  int bx = (plane + kBlock - 1) / kBlock;
-  if (!reduce) return dim3(bx, gf.NZ);
-  if (bx > kMaxRedBlocks) bx = kMaxRedBlocks;
-  int by = kMaxRedBlocks / bx;
-  if (by > gf.NZ) by = gf.NZ;
-  if (by < 1) by = 1;
-  return dim3(bx, by);
+// grid (y-x tiles, z chunks) and the planes per chunk; at most kMaxPartials blocks
+dim3 prolong_grid(const Geo& gf, int* zc) {
+  const int bx = (gf.NY * gf.NX + kBlock - 1) / kBlock;
+  int c = 4;
+  while ((long long)bx * ((gf.NZ + c - 1) / c) > kMaxPartials) c *= 2;
+  *zc = c;
+  return dim3(bx, (gf.NZ + c - 1) / c);
 }
 
 template <typename TX, typename TR, typename TD, typename TO>
@@ -193,15 +198,27 @@ void prolong_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, double 
                   const TD* diag, const uint8_t* nmaskF, TO* out, const RedWs* ws, double* sc, const int* fl) {
   int mode = 0;
   if (ws) mode = (fl && sc) ? 2 : 1;
-  const dim3 g = prolong_grid(gf, mode != 0);
+  int zc = 0;
+  const dim3 g = prolong_grid(gf, &zc);
   double* part = mode ? ws->partials : nullptr;
   if (span == 4)
-    hipLaunchKernelGGL((k_prolong<4, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, xc, rf, diag, nmaskF,
-                       out, part, fl);
+    hipLaunchKernelGGL((k_prolong<4, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, zc, xc, rf, diag,
+                       nmaskF, out, part, fl);
   else
-    hipLaunchKernelGGL((k_prolong<2, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, xc, rf, diag, nmaskF,
-                       out, part, fl);
+    hipLaunchKernelGGL((k_prolong<2, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, zc, xc, rf, diag,
+                       nmaskF, out, part, fl);
   if (mode) launch_fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, (int)(g.x * g.y), *ws, sc, (int*)fl, nullptr);
+}
+
+// x = omega * r / d on solid nodes (the 'Standard' cycle's pre-smoothing, 757)
+template <typename TR, typename TD, typename TO>
+__global__ __launch_bounds__(kBlock) void k_jacobi0(const TR* __restrict__ r, const TD* __restrict__ d,
+                                                    TO* __restrict__ x, double omega, long long n, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    const double dd = (double)d[i];
+    x[i] = dd != 0.0 ? (TO)(omega * ((double)r[i] / dd)) : TO(0);
+  }
 }
 
 }  // namespace
@@ -235,6 +252,29 @@ void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span
                            const double* rf, const float* diag, const uint8_t* nmaskF, float* out,
                            const RedWs* ws, double* sc, const int* fl) {
   prolong_impl(s, gf, gc, span, omega, xc, rf, diag, nmaskF, out, ws, sc, fl);
+}
+
+void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* xc, float* x,
+                        const uint8_t* nmaskF, const int* fl) {
+  int zc = 0;
+  const dim3 g = prolong_grid(gf, &zc);
+  if (span == 4)
+    hipLaunchKernelGGL((k_prolong<4, float, float, float, float, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
+                       (const float*)nullptr, (const float*)nullptr, nmaskF, x, (double*)nullptr, fl);
+  else
+    hipLaunchKernelGGL((k_prolong<2, float, float, float, float, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
+                       (const float*)nullptr, (const float*)nullptr, nmaskF, x, (double*)nullptr, fl);
+}
+
+void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
+                    const int* fl) {
+  hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
+                     n, fl);
+}
+void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
+                    const int* fl) {
+  hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
+                     n, fl);
 }
 
 }  // namespace t3x

@@ -13,7 +13,8 @@
 namespace t3x {
 
 constexpr int kBlock = 256;
-constexpr int kMaxRedBlocks = 2048;
+constexpr int kMaxRedBlocks = 2048;   // grid-stride reductions: blocks (x up to 8 values each)
+constexpr int kMaxPartials = 65536;   // partials buffer (doubles): one-value reductions over bigger grids
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -30,7 +30,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 class Params(C.Structure):
     _fields_ = [("E0", C.c_double), ("nu", C.c_double), ("Emin", C.c_double), ("penal", C.c_double),
                 ("tol", C.c_double), ("maxIT", C.c_int), ("omega", C.c_double), ("coarsestControl", C.c_int),
-                ("precision", C.c_int), ("checkEvery", C.c_int), ("timing", C.c_int)]
+                ("precision", C.c_int), ("checkEvery", C.c_int), ("timing", C.c_int), ("vcycle", C.c_int)]
 
 
 class Iter(C.Structure):
@@ -45,6 +45,23 @@ class Result(C.Structure):
                 ("tHist", C.c_void_p), ("xSnap", C.c_void_p), ("nSnap", C.c_int), ("resHist", C.c_void_p),
                 ("cSolid", C.c_double),
                 ("loops", C.c_int), ("warnings", C.c_int)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("memBudget", C.c_longlong), ("maxActive", C.c_int), ("params", Params)]
+
+
+class Request(C.Structure):
+    _fields_ = [("tag", C.c_longlong), ("nely", C.c_int), ("nelx", C.c_int), ("nelz", C.c_int),
+                ("mode", C.c_int), ("V0", C.c_double), ("nLoop", C.c_int), ("rMin", C.c_double),
+                ("rHat", C.c_double), ("loadDir", C.c_double * 3), ("loadMag", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_longlong), ("status", C.c_int), ("loops", C.c_int), ("pcgIts", C.c_int),
+                ("nE", C.c_int), ("tSubmit", C.c_double), ("tStart", C.c_double), ("tEnd", C.c_double),
+                ("cFinal", C.c_double), ("cSolid", C.c_double), ("footprint", C.c_longlong),
+                ("xPhys", C.POINTER(C.c_double)), ("cHist", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -83,6 +100,17 @@ def load_library():
     L.t3x_oc.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.t3x_mma.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp,
                           C.POINTER(C.c_int)]
+    L.t3x_footprint.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Params), C.c_int,
+                                C.POINTER(C.c_longlong)]
+    L.t3x_memory.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.t3x_stream_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
+    L.t3x_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
+    L.t3x_server_submit.argtypes = [vp, C.POINTER(Request)]
+    L.t3x_server_poll.argtypes = [vp, C.POINTER(Completion), C.c_int, C.c_int, C.POINTER(C.c_int)]
+    L.t3x_server_release.argtypes = [vp, C.c_longlong]
+    L.t3x_server_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_longlong)]
+    L.t3x_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -90,7 +118,9 @@ def load_library():
 EXPORTED = ["t3x_default_params", "t3x_create", "t3x_destroy", "t3x_last_error", "t3x_info", "t3x_run",
             "t3x_opt_begin", "t3x_opt_step", "t3x_opt_result", "t3x_kernel_time", "t3x_kernel_time_reset",
             "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance",
-            "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma"]
+            "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy",
+            "t3x_server_create", "t3x_server_submit", "t3x_server_poll", "t3x_server_release",
+            "t3x_server_stats", "t3x_server_destroy"]
 
 
 def default_params(**kw) -> Params:
@@ -237,6 +267,12 @@ class Problem:
         self._check(self.L.t3x_kernel_time(self.h, int(group), C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
+    def memory(self):
+        """Device bytes the problem holds (t3x_memory)."""
+        out = C.c_longlong(0)
+        self._check(self.L.t3x_memory(self.h, C.byref(out)))
+        return out.value
+
     def kernel_time_reset(self):
         self._check(self.L.t3x_kernel_time_reset(self.h))
 
@@ -283,3 +319,77 @@ def TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin, **params):
         return P.run(mode, V0, nLoop, rMin, rHat)
     finally:
         P.close()
+
+
+def footprint(nely, nelx, nelz, voxels=None, mode="GLOBAL", **params):
+    """Device bytes one problem allocates over a run (upper bound, t3x_footprint)."""
+    L = load_library()
+    P = default_params(**params)
+    out = C.c_longlong(0)
+    buf = None
+    if voxels is not None:
+        buf = np.ascontiguousarray(np.asarray(voxels, dtype=np.uint8).reshape(-1, order="F"))
+    m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
+    rc = L.t3x_footprint(None if buf is None else buf.ctypes.data, nely, nelx, nelz, C.byref(P), m, C.byref(out))
+    if rc != T3X_OK:
+        raise T3XError(f"t3x_footprint failed ({_STATUS.get(rc, rc)})")
+    return out.value
+
+
+def stream_copy_ms(nbytes, reps=20):
+    """Milliseconds per device-to-device copy of nbytes (t3x_stream_copy)."""
+    L = load_library()
+    ms = C.c_double(0.0)
+    rc = L.t3x_stream_copy(int(nbytes), int(reps), C.byref(ms))
+    if rc != T3X_OK:
+        raise T3XError(f"t3x_stream_copy failed ({_STATUS.get(rc, rc)})")
+    return ms.value
+
+
+class Server:
+    """Continuous-admission serving engine (include/t3x_serve.h)."""
+
+    def __init__(self, max_active=4, mem_budget=0, **params):
+        L = load_library()
+        cfg = ServerConfig(mem_budget, max_active, default_params(**params))
+        h = C.c_void_p()
+        rc = L.t3x_server_create(C.byref(cfg), C.byref(h))
+        if rc != T3X_OK:
+            raise T3XError(f"t3x_server_create failed ({_STATUS.get(rc, rc)})")
+        self.L, self.h = L, h
+
+    def submit(self, tag, nely, nelx, nelz, mode="GLOBAL", V0=0.12, nLoop=30, rMin=3 ** 0.5, rHat=0.0,
+               load_dir=(0.0, 0.0, -1.0), load_mag=1.0):
+        m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
+        r = Request(tag, nely, nelx, nelz, m, V0, nLoop, rMin, rHat, (C.c_double * 3)(*load_dir), load_mag)
+        rc = self.L.t3x_server_submit(self.h, C.byref(r))
+        if rc != T3X_OK:
+            raise T3XError(f"t3x_server_submit failed ({_STATUS.get(rc, rc)})")
+
+    def poll(self, max_n=64, wait_ms=100):
+        """Finished requests as dicts (results copied, server memory released)."""
+        buf = (Completion * max_n)()
+        n = C.c_int(0)
+        self.L.t3x_server_poll(self.h, buf, max_n, wait_ms, C.byref(n))
+        out = []
+        for c in buf[:n.value]:
+            d = dict(tag=c.tag, status=c.status, loops=c.loops, pcgIts=c.pcgIts, nE=c.nE, tSubmit=c.tSubmit,
+                     tStart=c.tStart, tEnd=c.tEnd, cFinal=c.cFinal, cSolid=c.cSolid, footprint=c.footprint,
+                     xPhys=np.ctypeslib.as_array(c.xPhys, (c.nE,)).copy() if c.nE else np.zeros(0),
+                     cHist=np.ctypeslib.as_array(c.cHist, (c.loops,)).copy() if c.loops else np.zeros(0))
+            self.L.t3x_server_release(self.h, c.tag)
+            out.append(d)
+        return out
+
+    def stats(self):
+        q, r, rd, m = C.c_int(), C.c_int(), C.c_int(), C.c_longlong()
+        self.L.t3x_server_stats(self.h, C.byref(q), C.byref(r), C.byref(rd), C.byref(m))
+        return dict(queued=q.value, running=r.value, ready=rd.value, mem_in_use=m.value)
+
+    def close(self):
+        if self.h:
+            self.L.t3x_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
