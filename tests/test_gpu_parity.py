@@ -161,12 +161,23 @@ SMALL_TOL = 1e-4
 
 @pytest.mark.parametrize("name", ["cuboid_6x12x6", "irregular_5x9x5"])
 def test_run_global_small(oracle_lib, name):
+    # PCG iteration counts on these tiny models are chaotic in FP64: the
+    # oracle's own coarsest-ordering variants differ by up to 4 iterations on
+    # irregular_5x9x5 (38/35/34 at design iteration 4); the bar is that
+    # measured spread + 1 (at least 3)
     g, o = make(name)
     rg = g.run("GLOBAL", 0.12, 10, math.sqrt(3), nSnap=10)
     ro = o.run("GLOBAL", 0.12, 10, math.sqrt(3), nSnap=10)
     g.close()
+    vox, fix, load = CASES[name]()
+    its = [ro["lssIts"]]
+    for v in (1, 2):
+        ov = Oracle(vox) if fix is None else Oracle(vox, fix, load)
+        ov.set_order_variant(v)
+        its.append(ov.run("GLOBAL", 0.12, 10, math.sqrt(3))["lssIts"])
+    spread = int(np.ptp(np.array(its), axis=0).max())
     assert rg["loops"] == ro["loops"]
-    assert np.abs(rg["lssIts"] - ro["lssIts"]).max() <= 3
+    assert np.abs(rg["lssIts"] - ro["lssIts"]).max() <= max(3, spread + 1)
     assert abs(rg["cSolid"] / ro["cSolid"] - 1) < 1e-7
     np.testing.assert_allclose(rg["cHist"], ro["cHist"], rtol=SMALL_TOL)
     assert np.abs(rg["xSnap"] - ro["xSnap"]).max() < SMALL_TOL

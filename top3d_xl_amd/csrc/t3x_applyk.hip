@@ -183,120 +183,68 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restri
 }
 
 // Coarse-level K*u with the stored Galerkin cell matrices (Solving_KbyU_MatrixFree
-// for iLevel > 1, 727-744), same z-march as above: the thread of node column
-// (y, x) reads, for each of the <= 4 cells around the column in cell layer k,
-// the 6 rows of the cell's 24x24 matrix that belong to its two nodes (upper
-// triangle, entry-major: a wave reads one entry of consecutive cells), so
-// every stored entry is read about once.  Coarse levels carry no fixed DOFs
-// (the boundary conditions are inside the Galerkin matrices) and vectors are
-// zero on void nodes.  FP32 throughout (the 'Standard' V-cycle smoother).
-template <int CY, int CX>
-__device__ __forceinline__ void ks_cell_pair(const float* __restrict__ Ks, long long NN, long long cell,
-                                             const float (&n0)[27], const float (&n1)[27], float (&a0)[3],
-                                             float (&a1)[3]) {
-  constexpr int dy = 1 - CY, dx = 1 - CX;
-  constexpr int jb = loc_index(dy, dx, 0), jt = loc_index(dy, dx, 1);
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int ky = CY + kLocDy[j], kx = CX + kLocDx[j];
-    const int nb = ky + 3 * kx;
-    const float* src = kLocDz[j] ? n1 : n0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const float v = src[nb * 3 + c];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        a0[r] += Ks[(long long)tri_index(3 * jb + r, 3 * j + c) * NN + cell] * v;
-        a1[r] += Ks[(long long)tri_index(3 * jt + r, 3 * j + c) * NN + cell] * v;
-      }
-    }
-  }
-}
-
+// for iLevel > 1, 727-744): one thread per node (coarse levels are small, so
+// the grid is the parallelism), holding its 27-node neighbourhood in
+// registers and reading, from each of its <= 8 cells, the 3 rows of the
+// cell's 24x24 matrix that belong to it (upper triangle, entry-major: a wave
+// reads one entry of consecutive cells), so every stored entry is read about
+// twice overall.  Coarse levels carry no fixed DOFs (the boundary conditions
+// are inside the Galerkin matrices) and vectors are zero on void nodes.
+// FP32 throughout (the 'Standard' V-cycle smoother).
 template <int EPI>
-__global__ __launch_bounds__(AK_T) void k_apply_Ks_tiled(Geo g, const float* __restrict__ Ks,
-                                                         const float* __restrict__ u, float* __restrict__ y,
-                                                         const float* __restrict__ rv, const float* __restrict__ dv,
-                                                         double omega, const int* fl) {
+__global__ __launch_bounds__(kBlock) void k_apply_Ks_node(Geo g, const float* __restrict__ Ks,
+                                                          const float* __restrict__ u, float* __restrict__ y,
+                                                          const float* __restrict__ rv, const float* __restrict__ dv,
+                                                          double omega, const int* fl) {
   if (fl && fl[FL_DONE]) return;
-  __shared__ float sU[2][3][AK_HX][AK_HY];
-  const int tid = threadIdx.x;
-  const int ty = tid % AK_TY, tx = tid / AK_TY;
-  const int y0 = blockIdx.x * AK_TY, x0 = blockIdx.y * AK_TX;
-  const int zc0 = blockIdx.z * AK_ZC;
-  const int zc1 = min(g.NZ, zc0 + AK_ZC);
-  const int yy = y0 + ty, xx = x0 + tx;
-  const bool own = yy < g.NY && xx < g.NX;
+  const int plane = g.NY * g.NX;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= plane) return;
+  const int x = idx / g.NY, yy = idx - x * g.NY;
+  const int z = blockIdx.y;
   const long long NN = g.NN;
-  const long long plane = (long long)g.NY * g.NX;
-  auto stage = [&](int z, int b) {
-    for (int t = tid; t < AK_HX * AK_HY; t += AK_T) {
-      const int hy = t % AK_HY, hx = t / AK_HY;
-      const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
-      float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-      if (z >= 0 && z < g.NZ && gy >= 0 && gx >= 0 && gy < g.NY && gx < g.NX) {
-        const long long n = gy + (long long)g.NY * gx + plane * z;
-        v0 = u[n];
-        v1 = u[NN + n];
-        v2 = u[2 * NN + n];
-      }
-      sU[b][0][hx][hy] = v0;
-      sU[b][1][hx][hy] = v1;
-      sU[b][2][hx][hy] = v2;
-    }
-  };
-  auto read_nb = [&](int b, float (&nbv)[27]) {
+  const long long n = idx + (long long)plane * z;
+  // neighbourhood nb[(ky + 3 kx + 9 kz) * 3 + c] = u(y-1+ky, x-1+kx, z-1+kz)
+  float nb[81];
+#pragma unroll
+  for (int kz = 0; kz < 3; ++kz)
 #pragma unroll
     for (int kx = 0; kx < 3; ++kx)
 #pragma unroll
-      for (int ky = 0; ky < 3; ++ky)
+      for (int ky = 0; ky < 3; ++ky) {
+        const int gy = yy - 1 + ky, gx = x - 1 + kx, gz = z - 1 + kz;
+        const bool in = gy >= 0 && gx >= 0 && gz >= 0 && gy < g.NY && gx < g.NX && gz < g.NZ;
+        const long long m = in ? gy + (long long)g.NY * gx + (long long)plane * gz : n;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nbv[(ky + 3 * kx) * 3 + c] = sU[b][c][tx + kx][ty + ky];
-  };
-  float n0[27], n1[27];
-  float a0[3] = {0.f, 0.f, 0.f}, a1[3] = {0.f, 0.f, 0.f};
-  const int kBeg = zc0 - 1;
-  stage(kBeg, kBeg & 1);
-  __syncthreads();
-  read_nb(kBeg & 1, n0);
-  stage(kBeg + 1, (kBeg + 1) & 1);
-  // the 4 cells around the column: (yy-1+CY, xx-1+CX), present if inside the cell grid
-  const bool c00 = own && yy - 1 >= 0 && xx - 1 >= 0 && yy - 1 < g.ny && xx - 1 < g.nx;
-  const bool c10 = own && yy >= 0 && xx - 1 >= 0 && yy < g.ny && xx - 1 < g.nx;
-  const bool c01 = own && yy - 1 >= 0 && xx >= 0 && yy - 1 < g.ny && xx < g.nx;
-  const bool c11 = own && yy < g.ny && xx < g.nx;
-  for (int k = kBeg; k < zc1; ++k) {
-    __syncthreads();
-    const int bk1 = (k + 1) & 1;
-    read_nb(bk1, n1);
-    __syncthreads();
-    if (k + 2 <= zc1) stage(k + 2, k & 1);
-    if (k >= 0 && k < g.nz) {
-      const long long base = (long long)yy + (long long)g.NY * xx + plane * k;  // cell (yy, xx, k)
-      if (c00) ks_cell_pair<0, 0>(Ks, NN, base - 1 - g.NY, n0, n1, a0, a1);
-      if (c10) ks_cell_pair<1, 0>(Ks, NN, base - g.NY, n0, n1, a0, a1);
-      if (c01) ks_cell_pair<0, 1>(Ks, NN, base - 1, n0, n1, a0, a1);
-      if (c11) ks_cell_pair<1, 1>(Ks, NN, base, n0, n1, a0, a1);
-    }
-    if (k >= zc0 && own) {
-      const long long n = yy + (long long)g.NY * xx + plane * k;
+        for (int c = 0; c < 3; ++c) nb[(ky + 3 * kx + 9 * kz) * 3 + c] = in ? u[c * NN + m] : 0.f;
+      }
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int cc = 0; cc < 8; ++cc) {
+    const int CY = cc & 1, CX = (cc >> 1) & 1, CZ = cc >> 2;
+    const int cy = yy - 1 + CY, cx = x - 1 + CX, cz = z - 1 + CZ;
+    if (cy < 0 || cx < 0 || cz < 0 || cy >= g.ny || cx >= g.nx || cz >= g.nz) continue;
+    const long long cell = cy + (long long)g.NY * cx + (long long)plane * cz;
+    const int jl = loc_index(1 - CY, 1 - CX, 1 - CZ);  // this node's local index in the cell
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = (CY + kLocDy[j]) + 3 * (CX + kLocDx[j]) + 9 * (CZ + kLocDz[j]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        if (EPI == EPI_RESID) {
-          y[c * NN + n] = rv[c * NN + n] - a0[c];
-        } else {
-          const float dc = dv[c * NN + n];
-          y[c * NN + n] = dc != 0.f ? u[c * NN + n] + float(omega) * (rv[c * NN + n] - a0[c]) / dc : 0.f;
-        }
+        const float v = nb[k * 3 + c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) acc[r] += Ks[(long long)tri_index(3 * jl + r, 3 * j + c) * NN + cell] * v;
       }
     }
+  }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      a0[r] = a1[r];
-      a1[r] = 0.f;
+  for (int c = 0; c < 3; ++c) {
+    if (EPI == EPI_RESID) {
+      y[c * NN + n] = rv[c * NN + n] - acc[c];
+    } else {
+      const float dc = dv[c * NN + n];
+      y[c * NN + n] = dc != 0.f ? nb[13 * 3 + c] + float(omega) * (rv[c * NN + n] - acc[c]) / dc : 0.f;
     }
-#pragma unroll
-    for (int i = 0; i < 27; ++i) n0[i] = n1[i];
   }
 }
 
@@ -330,15 +278,17 @@ void launch_fine_smooth32(hipStream_t s, const Geo& g, const float* Ke, const fl
 }
 
 
+static dim3 node_grid(const Geo& g) { return dim3((g.NY * g.NX + kBlock - 1) / kBlock, g.NZ); }
+
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
                            const int* fl) {
-  hipLaunchKernelGGL(k_apply_Ks_tiled<EPI_RESID>, applyk_grid(g), dim3(AK_T), 0, s, g, Ks, x, d, r,
+  hipLaunchKernelGGL(k_apply_Ks_node<EPI_RESID>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, d, r,
                      (const float*)nullptr, 0.0, fl);
 }
 
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
                             const float* diag, double omega, float* out, const int* fl) {
-  hipLaunchKernelGGL(k_apply_Ks_tiled<EPI_SMOOTH>, applyk_grid(g), dim3(AK_T), 0, s, g, Ks, x, out, r, diag, omega,
+  hipLaunchKernelGGL(k_apply_Ks_node<EPI_SMOOTH>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, out, r, diag, omega,
                      fl);
 }
 
