@@ -264,20 +264,6 @@ void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const d
                      (const double*)nullptr, 0.0);
 }
 
-void launch_fine_resid32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                         const float* x, const double* r, float* d, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_tiled<float, float, float, float, EPI_RESID, double, float>), applyk_grid(g),
-                     dim3(AK_T), 0, s, g, Ke, E, fix, x, d, (double*)nullptr, fl, 0, r, (const float*)nullptr, 0.0);
-}
-
-void launch_fine_smooth32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                          const float* x, const double* r, const float* diag, double omega, float* out,
-                          double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_tiled<float, float, float, float, EPI_SMOOTH, double, float>), applyk_grid(g),
-                     dim3(AK_T), 0, s, g, Ke, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag, omega);
-}
-
-
 static dim3 node_grid(const Geo& g) { return dim3((g.NY * g.NX + kBlock - 1) / kBlock, g.NZ); }
 
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,

@@ -716,7 +716,7 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
   launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl);                      // 757
-  launch_fine_resid32(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.d32, fl);                   // 761
+  launch_fine_resid32_mfma(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.d32, fl);              // 761
   launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const float*)P.d32, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
@@ -736,9 +736,9 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
     std::swap(V.xf, V.tf);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, P.lv[1].xf, P.x32, P.lv[0].nmask, fl);
-  launch_fine_smooth32(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z, fuse ? P.ws.partials : nullptr,
-                       fl);
-  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_tiled_blocks(g0), P.ws, sc, (int*)fl, nullptr);
+  launch_fine_smooth32_mfma(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z,
+                            fuse ? P.ws.partials : nullptr, fl);
+  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_mfma_blocks(g0), P.ws, sc, (int*)fl, nullptr);
   P.tend(TG_VCYCLE, ev);
 }
 

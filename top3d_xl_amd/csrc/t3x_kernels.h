@@ -32,6 +32,13 @@ enum FinOp { FIN_PQ = 0, FIN_ZR_INIT, FIN_ZR_LOOP, FIN_RR, FIN_SUM, FIN_SQRT };
 void launch_fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out);
 
 // ---------------------------------------------------------------- FEM
+// matrix-core FP32 K*u of the 'Standard' smoother (t3x_applyk_mfma.hip); partials: one per work-group
+int applyk_mfma_blocks(const Geo& g);
+void launch_fine_resid32_mfma(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                              const float* x, const double* r, float* d, const int* fl);
+void launch_fine_smooth32_mfma(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
+                               const float* x, const double* r, const float* diag, double omega, float* out,
+                               double* partials, const int* fl);
 // tiled K*u (t3x_applyk.hip); partials: one per work-group (nullptr: none)
 int applyk_tiled_blocks(const Geo& g);
 void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
@@ -80,14 +87,8 @@ void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, do
                     const int* fl);
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
                     const int* fl);
-// FP32 K*u smoothers (t3x_applyk.hip): fine level with E*Ke and BC rows, coarse
-// levels with the stored Galerkin matrices (FP32 copy).  resid: d = r - Kx;
-// smooth: out = x + omega (r - Kx) / diag (fine: fused out.r partials).
-void launch_fine_resid32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                         const float* x, const double* r, float* d, const int* fl);
-void launch_fine_smooth32(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                          const float* x, const double* r, const float* diag, double omega, float* out,
-                          double* partials, const int* fl);
+// FP32 K*u smoothers of coarse levels with the stored Galerkin matrices (FP32
+// copy; t3x_applyk.hip).  resid: d = r - Kx; smooth: out = x + omega (r - Kx) / diag.
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
                            const int* fl);
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
