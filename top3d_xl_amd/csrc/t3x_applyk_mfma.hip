@@ -87,31 +87,46 @@ __global__ __launch_bounds__(MK_T) void k_apply_K_mfma(Geo g, const T* __restric
       const int i = 16 * mb + (lane & 15), k = 4 * kb + (lane >> 4);
       ka[mb][kb] = i < 24 ? Ke[i * 24 + k] : T(0);
     }
-  // this lane's B column: dof 4 kb + lane/16 of cell (cy = lane & 15, cx)
-  auto stage = [&](int z, int b) {
-    for (int t = tid; t < MK_HT * MK_HT; t += MK_T) {
+  // ---- staging, split into global loads (issued a layer ahead, into
+  // registers) and LDS stores (after the matrix-core phase has released the
+  // buffer), so the load latency overlaps the MFMAs.
+  constexpr int NST = (MK_HT * MK_HT + MK_T - 1) / MK_T;  // staged nodes per thread (2)
+  T pu[NST][3], pe;
+  auto load_stage = [&](int z) {
+#pragma unroll
+    for (int s2 = 0; s2 < NST; ++s2) {
+      const int t = tid + s2 * MK_T;
       const int hy = t % MK_HT, hx = t / MK_HT;
       const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
-      T v0 = 0, v1 = 0, v2 = 0;
-      if (z >= 0 && z < g.NZ && gy >= 0 && gx >= 0 && gy < g.NY && gx < g.NX) {
+      pu[s2][0] = pu[s2][1] = pu[s2][2] = T(0);
+      if (t < MK_HT * MK_HT && z >= 0 && z < g.NZ && gy >= 0 && gx >= 0 && gy < g.NY && gx < g.NX) {
         const long long n = gy + (long long)g.NY * gx + plane * z;
         const uint8_t f = fix[n];
-        v0 = (f & 1) ? T(0) : T(u[n]);
-        v1 = (f & 2) ? T(0) : T(u[NN + n]);
-        v2 = (f & 4) ? T(0) : T(u[2 * NN + n]);
+        pu[s2][0] = (f & 1) ? T(0) : T(u[n]);
+        pu[s2][1] = (f & 2) ? T(0) : T(u[NN + n]);
+        pu[s2][2] = (f & 4) ? T(0) : T(u[2 * NN + n]);
       }
-      sU[b][0][hx][hy] = v0;
-      sU[b][1][hx][hy] = v1;
-      sU[b][2][hx][hy] = v2;
     }
-    for (int t = tid; t < MK_CT * MK_CT; t += MK_T) {
-      const int ey = t % MK_CT, ex = t / MK_CT;
+    {
+      const int ey = tid % MK_CT, ex = tid / MK_CT;  // MK_CT * MK_CT == MK_T
       const int gy = y0 - 1 + ey, gx = x0 - 1 + ex;
-      T e = 0;
+      pe = T(0);
       if (z >= 0 && z < g.nz && gy >= 0 && gx >= 0 && gy < g.ny && gx < g.nx)
-        e = T(E[gy + (long long)g.NY * gx + plane * z]);
-      sE[b][ex][ey] = e;
+        pe = T(E[gy + (long long)g.NY * gx + plane * z]);
     }
+  };
+  auto store_stage = [&](int b) {
+#pragma unroll
+    for (int s2 = 0; s2 < NST; ++s2) {
+      const int t = tid + s2 * MK_T;
+      if (t < MK_HT * MK_HT) {
+        const int hy = t % MK_HT, hx = t / MK_HT;
+        sU[b][0][hx][hy] = pu[s2][0];
+        sU[b][1][hx][hy] = pu[s2][1];
+        sU[b][2][hx][hy] = pu[s2][2];
+      }
+    }
+    sE[b][tid / MK_CT][tid % MK_CT] = pe;
   };
   // node thread of the gather: (ty, tx) in 15 x 15
   const int ty = tid % 16, tx = tid / 16;
@@ -121,9 +136,28 @@ __global__ __launch_bounds__(MK_T) void k_apply_K_mfma(Geo g, const T* __restric
   T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
   double pq = 0.0;
   const int kBeg = zc0 - 1;
-  stage(kBeg, kBeg & 1);
-  stage(kBeg + 1, (kBeg + 1) & 1);
+  load_stage(kBeg);
+  store_stage(kBeg & 1);
+  load_stage(kBeg + 1);
+  store_stage((kBeg + 1) & 1);
   for (int k = kBeg; k < zc1; ++k) {
+    const bool more = k + 2 <= zc1;
+    if (more) load_stage(k + 2);  // in flight during this layer's MFMAs
+    // epilogue operands of node layer k, also loaded ahead
+    const bool epi = k >= zc0 && own;
+    const long long n = yy + (long long)g.NY * xx + plane * k;
+    uint8_t f = 0;
+    double uo[3] = {0.0, 0.0, 0.0}, ro[3] = {0.0, 0.0, 0.0};
+    T dd[3] = {T(0), T(0), T(0)};
+    if (epi) {
+      f = fix[n];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        uo[c] = u[c * NN + n];
+        if (EPI != MEPI_KU) ro[c] = double(rv[c * NN + n]);
+        if (EPI == MEPI_SMOOTH) dd[c] = T(dv[c * NN + n]);
+      }
+    }
     __syncthreads();  // node layers k, k+1 and cell layer k's moduli staged; sY free
     const int bk = k & 1, bk1 = (k + 1) & 1;
     // ---- matrix-core phase: cells (cy, cx), cx = 4 wave .. 4 wave + 3
@@ -149,7 +183,7 @@ __global__ __launch_bounds__(MK_T) void k_apply_K_mfma(Geo g, const T* __restric
       }
     }
     __syncthreads();  // sY complete; node layer k's buffer free
-    if (k + 2 <= zc1) stage(k + 2, bk);
+    if (more) store_stage(bk);
     // ---- gather: rows of this node (bottom: layer k, top: layer k+1) from its 4 cells
     if (gnode) {
 #pragma unroll
@@ -164,24 +198,20 @@ __global__ __launch_bounds__(MK_T) void k_apply_K_mfma(Geo g, const T* __restric
         }
       }
     }
-    if (k >= zc0 && own) {  // node layer k complete
-      const long long n = yy + (long long)g.NY * xx + plane * k;
-      const uint8_t f = fix[n];
+    if (epi) {  // node layer k complete
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double uc = u[c * NN + n];
-        const T ku = ((f >> c) & 1) ? T(uc) : a0[c];
+        const T ku = ((f >> c) & 1) ? T(uo[c]) : a0[c];
         if (EPI == MEPI_KU) {
           const TY v = TY(ku);
           y[c * NN + n] = v;
-          pq += uc * double(v);
+          pq += uo[c] * double(v);
         } else if (EPI == MEPI_RESID) {
-          y[c * NN + n] = TY(T(rv[c * NN + n]) - ku);
+          y[c * NN + n] = TY(T(ro[c]) - ku);
         } else {
-          const T rc = T(rv[c * NN + n]), dc = T(dv[c * NN + n]);
-          const TY v = dc != T(0) ? TY(T(uc) + T(omega) * (rc - ku) / dc) : TY(0);
+          const TY v = dd[c] != T(0) ? TY(T(uo[c]) + T(omega) * (T(ro[c]) - ku) / dd[c]) : TY(0);
           y[c * NN + n] = v;
-          pq += double(v) * double(rv[c * NN + n]);
+          pq += double(v) * ro[c];
         }
       }
     }
