@@ -45,17 +45,30 @@ VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=1, vcycle=1, omeg
             "mixed_adapted": dict(precision=1)}
 
 
-def gpu_run(args, params):
+def make_model(args):
+    """The workload's voxels and boundary conditions (None: built-in cuboid BCs)."""
     import numpy as np
+    ny, nx, nz = parse_size(args.size)
+    if args.workload == "cuboid":
+        return np.ones((ny, nx, nz), dtype=bool), None, 0.12
+    from top3d_xl_amd.irregular import generate
+    g = generate(args.seed, (ny, nx, nz))
+    return g["voxels"], g, 0.4
+
+
+def gpu_run(args, params, model):
     import torch
 
     import top3d_xl_amd as t3x
 
-    ny, nx, nz = parse_size(args.size)
-    vox = np.ones((ny, nx, nz), dtype=bool)
-    P = t3x.Problem(vox, timing=1, checkEvery=args.check_every, **params)
+    vox, bc, V0 = model
+    if bc is None:
+        P = t3x.Problem(vox, timing=1, checkEvery=args.check_every, **params)
+    else:
+        P = t3x.Problem(vox, (bc["fix_nodes"], bc["fix_state"]), (bc["load_nodes"], bc["load_vec"]), timing=1,
+                        checkEvery=args.check_every, **params)
     nN, nE = P.nN, P.nE
-    P.opt_begin("GLOBAL", 0.12, args.warmup + args.steps, math.sqrt(3))
+    P.opt_begin("GLOBAL", V0, args.warmup + args.steps, math.sqrt(3))
     warm = []
     for _ in range(args.warmup):
         rec, _ = P.opt_step()
@@ -109,7 +122,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", default="256x512x256", help="nely x nelx x nelz")
+    ap.add_argument("--size", default=None, help="nely x nelx x nelz (default 256x512x256; irregular 384x768x384)")
+    ap.add_argument("--workload", default="cuboid", choices=["cuboid", "irregular"],
+                    help="cuboid: built-in BCs, V0 0.12; irregular: seeded ellipsoid-union domain, V0 0.4")
+    ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--cpu-iters", type=int, default=2)
@@ -121,11 +137,14 @@ def main():
                                    "multi-GPU design (x-slab decomposition) is documented, not run",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.size is None:
+        args.size = "256x512x256" if args.workload == "cuboid" else "384x768x384"
+    model = make_model(args)
     runs = {}
     names = {"fp64": ["fp64"], "mixed": ["mixed"], "both": ["fp64", "mixed", "mixed_adapted"]}[args.precision]
     for name in names:
         print(f"[bench] {name}: {VARIANTS[name]}", file=sys.stderr, flush=True)
-        runs[name] = gpu_run(args, VARIANTS[name])
+        runs[name] = gpu_run(args, VARIANTS[name], model)
     head = "mixed" if "mixed" in runs else "fp64"
     g = runs[head]
     ny, nx, nz = parse_size(args.size)
@@ -168,9 +187,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f64+f32",
-        "data": "synthetic (built-in cuboid model, no dataset)",
-        "config": {"workload": f"GLOBAL topology optimisation, cuboid {ny}x{nx}x{nz} voxels (nely x nelx x nelz), "
-                               f"{3 * g['nN']} DOF, V0=0.12, rMin=sqrt(3), built-in BCs",
+        "data": "synthetic (built-in cuboid model, no dataset)" if args.workload == "cuboid" else
+                "synthetic (seeded irregular domain, no dataset)",
+        "config": {"workload": (f"GLOBAL topology optimisation, cuboid {ny}x{nx}x{nz} voxels (nely x nelx x nelz), "
+                                f"{3 * g['nN']} DOF, V0=0.12, rMin=sqrt(3), built-in BCs")
+                   if args.workload == "cuboid" else
+                   (f"GLOBAL topology optimisation, irregular domain (seed {args.seed}: 40 ellipsoids in a "
+                    f"{ny}x{nx}x{nz} box intersected with a cuboid shell, largest component, "
+                    f"{g['nE']} solid voxels), {3 * g['nN']} DOF, V0=0.4, rMin=sqrt(3), bottom-slab fixed, "
+                    f"unit load on top-slab surface nodes"),
                    "precision": head if head == "fp64" else
                    "mixed: FP64 PCG (K*p, vectors, dots) + FP32 'Standard' V(1,1) cycle (omega 0.4) "
                    "around the FP64 coarsest Cholesky"},
