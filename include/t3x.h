@@ -77,6 +77,12 @@ int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, co
                const t3x_params* params, t3x_problem** out);
 void t3x_destroy(t3x_problem* h);
 const char* t3x_last_error(const t3x_problem* h);
+/* Passive elements (passiveElements_, 472, 524-527): voxel ids (0-based,
+ * column-major, unpadded nely x nelx x nelz grid) held solid: OC sets them to
+ * 1 (150, 165), LOCAL excludes them from MMA and sets x and xPhys to 1
+ * (274-276, 366, 381).  Call before t3x_run / t3x_opt_begin; void voxels are
+ * ignored; replaces any earlier set. */
+int t3x_set_passive(t3x_problem* h, int n, const int* voxelIds);
 /* info[0]=numLevels; per level l at 1+8l: resX,resY,resZ,span,numElements,numNodes,0,0 */
 int t3x_info(const t3x_problem* h, int* info);
 

@@ -18,8 +18,8 @@ function TOP3D_XL(inputModel, consType, V0, nLoop, rMin, varargin)
 	params = struct('precision', 0); %% 0: FP64 plain path (1: mixed precision)
 	if ischar(inputModel) || isstring(inputModel)
 		M = ReadTopVoxel(inputModel);
-		if ~isempty(M.passive), warning('Passive elements are not applied by this build'); end
 		h = t3x_mex('create', M.voxels, M.fix(:,1), M.fix(:,2:4), M.load(:,1), M.load(:,2:4), params);
+		if ~isempty(M.passive), t3x_mex('passive', h, M.passive); end
 	elseif islogical(inputModel)
 		h = t3x_mex('create', inputModel, params);
 	else

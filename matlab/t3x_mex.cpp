@@ -10,6 +10,7 @@
 //                                                      explicit BCs, 1-based node ids of the
 //                                                      unpadded (nely+1)x(nelx+1)x(nelz+1) grid
 //   h = t3x_mex(..., params)                           optional struct with t3x_params fields
+//   t3x_mex('passive', h, voxelIds)                   passive elements, 1-based voxel ids
 //   out = t3x_mex('run', h, consType, V0, nLoop, rMin [, rHat])
 //   t3x_mex('destroy', h)
 //
@@ -196,6 +197,15 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     cmd_create(nlhs, plhs, nrhs, prhs);
   } else if (!std::strcmp(cmd, "run")) {
     cmd_run(nlhs, plhs, nrhs, prhs);
+  } else if (!std::strcmp(cmd, "passive")) {
+    if (nrhs < 3) mexErrMsgIdAndTxt("t3x:input", "Wrong Input!");
+    t3x_problem* h = handle_arg(prhs[1]);
+    const size_t n = mxGetNumberOfElements(prhs[2]);
+    const double* v = mxGetDoubles(prhs[2]);
+    std::vector<int> ids(n);
+    for (size_t i = 0; i < n; ++i) ids[i] = (int)v[i] - 1;  // 1-based -> 0-based
+    const int rc = t3x_set_passive(h, (int)n, ids.data());
+    if (rc != T3X_OK) fail(rc, h);
   } else if (!std::strcmp(cmd, "destroy")) {
     if (nrhs < 2) mexErrMsgIdAndTxt("t3x:input", "Wrong Input!");
     t3x_problem* h = handle_arg(prhs[1]);

@@ -62,6 +62,7 @@ def _load():
                           C.POINTER(C.c_double), C.POINTER(C.c_int), _ip, _dp]
     L.orc_num_threads.restype = C.c_int
     L.orc_set_order_variant.argtypes = [vp, C.c_int]
+    L.orc_set_passive.argtypes = [vp, C.c_int, _ip]
     _lib = L
     return L
 
@@ -227,6 +228,11 @@ class Oracle:
     def set_order_variant(self, k):
         """k-th best coarsest-level ordering: same factor, different FP64 rounding."""
         self.L.orc_set_order_variant(self.h, int(k))
+
+    def set_passive(self, voxel_ids):
+        """passiveElements_: 0-based voxel ids of the unpadded grid."""
+        ids = np.ascontiguousarray(voxel_ids, dtype=np.int32)
+        self.L.orc_set_passive(self.h, len(ids), ids)
 
     @staticmethod
     def num_threads():

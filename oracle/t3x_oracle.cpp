@@ -269,6 +269,7 @@ struct Model {
   std::vector<int> loadNode;
   std::vector<double> loadVal;            // 3 per loaded node
   std::vector<double> F;                  // 3*numNodes
+  std::vector<char> isPassive;            // per compact element (passiveElements_, 81 / 274)
   bool bcOk = false;
   // BC element matrices, 1338-1437
   std::vector<int> mapUniqueKes;          // numElements: 0 or rank+1
@@ -1142,6 +1143,7 @@ double oc_update(const Model& M, std::vector<double>& x, const std::vector<doubl
       v = std::min(1.0, v);
       v = std::max(x[i] - move, v);
       xnew[i] = std::max(0.0, v);
+      if (!M.isPassive.empty() && M.isPassive[i]) xnew[i] = 1.0;  // 150
     }
     const double gt = fval + det_sum(n, [&](i64 i) { return 1.0 * (xnew[i] - x[i]); });  // 151
     if (gt > 0) l1 = lmid; else l2 = lmid;  // 152
@@ -1369,6 +1371,9 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
       fval = det_sum(nE, [&](i64 i) { return xPhys[i]; }) / (double)nE - V0;  // 145
       change = oc_update(M, x, xPhys, dc, V0, nullptr);
       xPhys = x;  // 160
+      if (!M.isPassive.empty())
+        for (int e = 0; e < nE; ++e)
+          if (M.isPassive[e]) xPhys[e] = 1.0;  // 165
       double t3 = now_s();
       tOpt += t3 - t2;
     } else {
@@ -1396,21 +1401,31 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
       fIts += filter_run(M, M.filt[0], tmp.data(), dfdx.data());  // 350
       double t5 = now_s();
       tFilt += t5 - t4;
-      // MMA (354-369); no passive elements -> every element is active
-      if (loop == 1) { mma.n = nE; mma.xold1 = x; mma.xold2 = x; mma.low.assign(nE, 0.0); mma.upp.assign(nE, 0.0); }
+      // MMA (354-369) over the active (non-passive) elements, 274-276
+      std::vector<int> act;
+      for (int e = 0; e < nE; ++e)
+        if (M.isPassive.empty() || !M.isPassive[e]) act.push_back(e);
+      const int na = (int)act.size();
+      std::vector<double> xa(na), dca(na), dfa(na);
+      for (int k = 0; k < na; ++k) { xa[k] = x[act[k]]; dca[k] = dc[act[k]]; dfa[k] = dfdx[act[k]]; }
+      if (loop == 1) { mma.n = na; mma.xold1 = xa; mma.xold2 = xa; mma.low.assign(na, 0.0); mma.upp.assign(na, 0.0); }
       fval = std::pow(sp, 1.0 / p) - 1.0;  // 360
-      std::vector<double> xmin(nE), xmax(nE);
-      for (int e = 0; e < nE; ++e) { xmin[e] = std::max(0.0, x[e] - 0.1); xmax[e] = std::min(1.0, x[e] + 0.1); }
-      mma_update(mma, loop, x.data(), xmin.data(), xmax.data(), dc.data(), fval, dfdx.data(), nullptr);
+      std::vector<double> xmin(na), xmax(na);
+      for (int k = 0; k < na; ++k) { xmin[k] = std::max(0.0, xa[k] - 0.1); xmax[k] = std::min(1.0, xa[k] + 0.1); }
+      mma_update(mma, loop, xa.data(), xmin.data(), xmax.data(), dca.data(), fval, dfa.data(), nullptr);
       change = 0.0;
-      for (int e = 0; e < nE; ++e) change = std::max(change, std::fabs(mma.x[e] - x[e]));  // 368
-      x = mma.x;  // 369
+      for (int k = 0; k < na; ++k) change = std::max(change, std::fabs(mma.x[k] - xa[k]));  // 368
+      if (na < nE) x.assign(nE, 1.0);  // 366: x = onesArrSingle; x(activeEles) = xmma
+      for (int k = 0; k < na; ++k) x[act[k]] = mma.x[k];  // 369
       double t6 = now_s();
       tOpt += t6 - t5;
       fIts += filter_run(M, M.filt[0], x.data(), xTilde.data());  // 374
       if (loopbeta >= 40) { beta = std::min(2 * beta, 128.0); loopbeta = 0; }  // 375-379
       const double d2 = std::tanh(beta * eta) + std::tanh(beta * (1 - eta));
       for (int e = 0; e < nE; ++e) xPhys[e] = (std::tanh(beta * eta) + std::tanh(beta * (xTilde[e] - eta))) / d2;  // 380
+      if (!M.isPassive.empty())
+        for (int e = 0; e < nE; ++e)
+          if (M.isPassive[e]) xPhys[e] = 1.0;  // 381
       tFilt += now_s() - t6;
     }
     sharp = 4.0 * det_sum(nE, [&](i64 i) { return xPhys[i] * (1.0 - xPhys[i]); }) / (double)nE;  // 166 / 382
@@ -1621,5 +1636,21 @@ int orc_num_threads(void) { return omp_get_max_threads(); }
 // Every ordering gives the same factor in exact arithmetic, as CHOLMOD's
 // fill-reducing permutation does; it only changes FP64 rounding.
 void orc_set_order_variant(void* h, int k) { ((Model*)h)->orderVariant = k; }
+
+// passiveElements_ (472, 524-527): 0-based voxel ids of the unpadded
+// nely x nelx x nelz grid, mapped to compact elements; void voxels ignored.
+void orc_set_passive(void* h, int n, const int* voxelIds) {
+  Model* M = (Model*)h;
+  const Level& L = M->L[0];
+  M->isPassive.assign(L.numElements, 0);
+  for (int i = 0; i < n; ++i) {
+    const int v = voxelIds[i];
+    if (v < 0 || v >= M->nely * M->nelx * M->nelz) continue;
+    const int iy = v % M->nely, ix = (v / M->nely) % M->nelx, iz = v / (M->nely * M->nelx);
+    const int dense = iy + L.resY * (ix + L.resX * iz);
+    const int c = L.eleMapForward[dense];
+    if (c > 0) M->isPassive[c - 1] = 1;
+  }
+}
 
 }  // extern "C"

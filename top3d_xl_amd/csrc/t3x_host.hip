@@ -160,7 +160,9 @@ struct t3x_problem {
   double* dKe = nullptr;
   double* dT2 = nullptr;
   uint8_t* fix = nullptr;
-  uint8_t* passive = nullptr;
+  uint8_t* passive = nullptr;  // passive cells (passiveElements_), or null
+  uint8_t* active = nullptr;   // solid and not passive (MMA design variables, 274-276)
+  int nPassive = 0;
   double* bvec = nullptr;
   double *F = nullptr, *U = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp3 = nullptr;
   double *E = nullptr, *x = nullptr, *xPhys = nullptr, *ce = nullptr, *dc = nullptr;
@@ -260,6 +262,7 @@ struct t3x_problem {
   double* volMax() { return cb; }
   long long NN() const { return lv[0].NN; }
   bool mixed() const { return P.precision == 1; }  // FP32 V-cycle
+  const uint8_t* design_mask() const { return active ? active : lv[0].emask; }
   bool standard() const { return P.vcycle == 1; }  // 'Standard' V-cycle (FP32)
   void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
 };
@@ -885,14 +888,14 @@ double oc_update(t3x_problem& P, double* x, const double* xPhys, const double* d
   std::memcpy(P.hbuf, init, sizeof(init));
   T3X_HIP(hipMemcpyAsync(P.oc, P.hbuf, sizeof(init), hipMemcpyHostToDevice, s));
   for (int pass = 0; pass < 64; ++pass) {
-    launch_oc_pass(s, P.NN(), P.lv[0].emask, nullptr, x, dc, P.ws, P.oc, 3);
+    launch_oc_pass(s, P.NN(), P.lv[0].emask, P.passive, x, dc, P.ws, P.oc, 3);
     if (pass % 8 == 7) {
       T3X_HIP(hipMemcpyAsync(P.hbuf, P.oc, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
       P.sync();
       if (P.hbuf[3] != 0.0) break;
     }
   }
-  launch_oc_final(s, P.NN(), P.lv[0].emask, nullptr, x, dc, P.ws, P.oc);
+  launch_oc_final(s, P.NN(), P.lv[0].emask, P.passive, x, dc, P.ws, P.oc);
   T3X_HIP(hipMemcpyAsync(P.hbuf, P.oc, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
   P.sync();
   if (nBis) *nBis = (int)P.hbuf[5];
@@ -914,11 +917,11 @@ int mma_update(t3x_problem& P, int iter, const double* dfdx, double gx, const do
   const long long NN = P.NN();
   P.hbuf[0] = gx;
   T3X_HIP(hipMemcpyAsync(P.mm + MM_GX, P.hbuf, sizeof(double), hipMemcpyHostToDevice, s));
-  launch_mma_gensub(s, NN, P.lv[0].emask, iter, P.mx, P.mxmin, P.mxmax, P.mxold1, P.mxold2, P.mlow, P.mupp, dfdx,
-                    dgdx, P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij, P.ws, P.mm, (double)P.nCells);
+  const uint8_t* am = P.design_mask();
+  launch_mma_gensub(s, NN, am, iter, P.mx, P.mxmin, P.mxmax, P.mxold1, P.mxold2, P.mlow, P.mupp, dfdx, dgdx,
+                    P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij, P.ws, P.mm, (double)(P.nCells - P.nPassive));
   for (int k = 0; k < 4000; ++k) {
-    launch_mma_step(s, NN, P.lv[0].emask, P.mx, P.mlow, P.mupp, P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij,
-                    P.ws, P.mm);
+    launch_mma_step(s, NN, am, P.mx, P.mlow, P.mupp, P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij, P.ws, P.mm);
     if (k % 4 == 3) {
       T3X_HIP(hipMemcpyAsync(P.hbuf, P.mm, MM_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
       P.sync();
@@ -1043,7 +1046,7 @@ int opt_step(t3x_problem& P, t3x_iter* it) {
     P.sync();
     const double t5 = wall();
     tFilt += t5 - t4;
-    // MMA (354-369); every cell is active (no passive cells in these models)
+    // MMA (354-369) over the active cells (274-276)
     if (S.loop == 1) {  // 355
       launch_copy(s, P.mxold1, P.x, NN);
       launch_copy(s, P.mxold2, P.x, NN);
@@ -1057,13 +1060,15 @@ int opt_step(t3x_problem& P, t3x_iter* it) {
     mma_update(P, S.loop, P.dc, fval, dfdx);  // 367
     launch_copy(s, tmp, P.x, NN);             // xval
     launch_copy(s, P.x, P.mx, NN);            // 369
-    launch_masked_maxdiff(s, NN, em, P.x, tmp, P.ws, P.sc + 33);  // 368
+    if (P.passive) launch_set_where(s, NN, P.passive, P.x, 1.0);  // 366: x(passive) = 1
+    launch_masked_maxdiff(s, NN, P.design_mask(), P.x, tmp, P.ws, P.sc + 33);  // 368
     S.change = read_scalar(P, P.sc + 33);
     const double t6 = wall();
     tOpt += t6 - t5;
     fIts += filter_run(P, 0, P.x, xTilde);                                          // 374
     if (S.loopbeta >= 40) { S.beta = std::min(2 * S.beta, 128.0); S.loopbeta = 0; }  // 375-379
     launch_elementwise(s, OP_HEAVI, NN, em, P.xPhys, xTilde, nullptr, S.beta, eta, 0);  // 380
+    if (P.passive) launch_set_where(s, NN, P.passive, P.xPhys, 1.0);                  // 381
     P.sync();
     tFilt += wall() - t6;
   }
@@ -1312,6 +1317,33 @@ int t3x_stream_copy(long long bytes, int reps, double* msPerCopy) {
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
   return rc;
+}
+
+int t3x_set_passive(t3x_problem* h, int n, const int* voxelIds) {
+  return guarded(h, [&] {
+    if (n < 0 || (n > 0 && !voxelIds)) { h->err = "t3x_set_passive: bad arguments"; return (int)T3X_ERR_INPUT; }
+    const Level& F = h->lv[0];
+    std::vector<uint8_t> pas(F.NN, 0), act(F.hEmask);
+    int cnt = 0;
+    for (int i = 0; i < n; ++i) {
+      const long long v = voxelIds[i];
+      if (v < 0 || v >= (long long)h->nely * h->nelx * h->nelz) continue;
+      const int iy = (int)(v % h->nely), ix = (int)((v / h->nely) % h->nelx), iz = (int)(v / ((long long)h->nely * h->nelx));
+      const long long d = iy + (long long)F.NY * (ix + (long long)F.NX * iz);
+      if (!F.hEmask[d] || pas[d]) continue;  // void voxels are not elements (527)
+      pas[d] = 1;
+      act[d] = 0;
+      ++cnt;
+    }
+    if (!h->passive) {
+      h->passive = h->alloc<uint8_t>(F.NN);
+      h->active = h->alloc<uint8_t>(F.NN);
+    }
+    T3X_HIP(hipMemcpy(h->passive, pas.data(), F.NN, hipMemcpyHostToDevice));
+    T3X_HIP(hipMemcpy(h->active, act.data(), F.NN, hipMemcpyHostToDevice));
+    h->nPassive = cnt;
+    return (int)T3X_OK;
+  });
 }
 
 int t3x_memory(const t3x_problem* h, long long* bytes) {

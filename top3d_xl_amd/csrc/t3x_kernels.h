@@ -150,6 +150,7 @@ void launch_oc_pass(hipStream_t s, long long n, const uint8_t* emask, const uint
                     const double* dc, const RedWs& ws, double* oc, int depth);
 void launch_oc_final(hipStream_t s, long long n, const uint8_t* emask, const uint8_t* passive, double* x,
                      const double* dc, const RedWs& ws, double* oc);
+void launch_set_where(hipStream_t s, long long n, const uint8_t* mask, double* a, double v);
 void launch_elementwise(hipStream_t s, int op, long long n, const uint8_t* emask, double* a, const double* b,
                         const double* c, double p0, double p1, double p2);
 enum { OP_DC = 0, OP_MUL, OP_DIVMAX, OP_HEAVI, OP_DHEAVI, OP_PNORMGRAD, OP_COPY, OP_SET, OP_DIVC,

@@ -1395,4 +1395,13 @@ void launch_stream_copy(hipStream_t s, double* dst, const double* src, long long
                      nDoubles / 2);
 }
 
+// a[i] = v where mask[i] (x(passiveElements) = 1, 165 / 366 / 381)
+__global__ void k_set_where(long long n, const uint8_t* __restrict__ mask, double* __restrict__ a, double v) {
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+    if (mask[i]) a[i] = v;
+}
+void launch_set_where(hipStream_t s, long long n, const uint8_t* mask, double* a, double v) {
+  hipLaunchKernelGGL(k_set_where, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, n, mask, a, v);
+}
+
 }  // namespace t3x

@@ -103,6 +103,7 @@ def load_library():
     L.t3x_footprint.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Params), C.c_int,
                                 C.POINTER(C.c_longlong)]
     L.t3x_memory.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.t3x_set_passive.argtypes = [vp, C.c_int, C.c_void_p]
     L.t3x_stream_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.t3x_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
     L.t3x_server_submit.argtypes = [vp, C.POINTER(Request)]
@@ -118,7 +119,7 @@ def load_library():
 EXPORTED = ["t3x_default_params", "t3x_create", "t3x_destroy", "t3x_last_error", "t3x_info", "t3x_run",
             "t3x_opt_begin", "t3x_opt_step", "t3x_opt_result", "t3x_kernel_time", "t3x_kernel_time_reset",
             "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance",
-            "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy",
+            "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy", "t3x_set_passive",
             "t3x_server_create", "t3x_server_submit", "t3x_server_poll", "t3x_server_release",
             "t3x_server_stats", "t3x_server_destroy"]
 
@@ -266,6 +267,12 @@ class Problem:
         ms = C.c_double(0.0)
         self._check(self.L.t3x_kernel_time(self.h, int(group), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def set_passive(self, voxel_ids):
+        """Passive elements: 0-based voxel ids of the unpadded grid (t3x_set_passive)."""
+        ids = np.ascontiguousarray(voxel_ids, dtype=np.int32)
+        self._keep.append(ids)
+        self._check(self.L.t3x_set_passive(self.h, len(ids), ids.ctypes.data if len(ids) else None))
 
     def memory(self):
         """Device bytes the problem holds (t3x_memory)."""
