@@ -54,25 +54,52 @@ struct BandView {
 };
 
 // ---- 64x64 tile helpers in LDS -----------------------------------------
-// In-place Cholesky of the lower triangle of sA: two barriers per column, no
-// integer division (thread t owns row t/4 and columns t%4, t%4+4, ...).
+// In-place Cholesky of the lower triangle of sA, blocked in 4 panels of 16
+// columns: the block's first wave factors a panel in registers (lane i owns
+// row i; the pivot and the column entries travel by cross-lane shuffles, no
+// LDS round-trip per element), then all threads apply the panel's rank-16
+// update to the trailing lower triangle.
 __device__ void tile_potrf(double (*sA)[TB + 1], int* status) {
-  const int row = threadIdx.x >> 2, c0 = threadIdx.x & 3;
-  for (int c = 0; c < TB; ++c) {
-    const double d = sqrt(sA[c][c]);  // every thread reads the same pivot
-    if (row > c && c0 == 0) sA[row][c] = sA[row][c] / d;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (!(d > 0.0)) *status = 1;
-      sA[c][c] = d;
-    }
-    if (row > c) {
-      const double lr = sA[row][c];
+  constexpr int PB = 16;
+  for (int b = 0; b < TB / PB; ++b) {
+    const int b0 = b * PB;
+    if (threadIdx.x < TB) {
+      const int i = threadIdx.x;
+      double p[PB];
 #pragma unroll
-      for (int q = 0; q < TB / 4; ++q) {
-        const int k = c0 + 4 * q;
-        if (k > c && k <= row) sA[row][k] -= lr * sA[k][c];
+      for (int k = 0; k < PB; ++k) p[k] = sA[i][b0 + k];
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        const int c = b0 + k;
+        const double piv = __shfl(p[k], c, 64);
+        const double d = sqrt(piv);
+        bad |= !(piv > 0.0);
+        if (i == c) p[k] = d;
+        else if (i > c) p[k] = p[k] / d;
+#pragma unroll
+        for (int kk = k + 1; kk < PB; ++kk) {
+          const double ljc = __shfl(p[k], b0 + kk, 64);  // L(b0 + kk, c)
+          if (i >= b0 + kk) p[kk] -= p[k] * ljc;
+        }
       }
+      if (i >= b0) {
+#pragma unroll
+        for (int k = 0; k < PB; ++k)
+          if (b0 + k <= i) sA[i][b0 + k] = p[k];
+      }
+      if (bad && i == b0) *status = 1;
+    }
+    __syncthreads();
+    // trailing update: rows and columns >= b0 + PB, lower triangle
+    const int t0 = b0 + PB, m = TB - t0;
+    for (int t = threadIdx.x; t < m * m; t += CT) {
+      const int i = t0 + t / m, j = t0 + t % m;
+      if (j > i) continue;
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) acc += sA[i][b0 + k] * sA[j][b0 + k];
+      sA[i][j] -= acc;
     }
     __syncthreads();
   }
@@ -135,9 +162,12 @@ __global__ __launch_bounds__(CT) void k_chol_diag(BandView V, int d0, int* statu
 }
 
 // L(i, J) = A(i, J) * L_JJ^{-T} for the row tiles below diagonal tile J, by
-// forward substitution (one row per thread, L_JJ in LDS): no tile inverse is
-// needed on the critical path.
+// forward substitution in 4 column blocks of 16: lane i solves its row's
+// block in registers (right-looking: independent FMAs after each division;
+// entries of L_JJ are uniform LDS broadcasts), then updates the row's later
+// columns in LDS with the block (a rank-16 update).
 __global__ __launch_bounds__(CT) void k_chol_trsm(BandView V, int J) {
+  constexpr int PB = 16;
   __shared__ double sL[TB][TB + 1];
   __shared__ double sX[TB][TB + 1];
   const int j0 = J * TB, r0 = j0 + TB;
@@ -150,10 +180,25 @@ __global__ __launch_bounds__(CT) void k_chol_trsm(BandView V, int J) {
   __syncthreads();
   if (threadIdx.x < TB) {
     const int i = threadIdx.x;
-    for (int c = 0; c < TB; ++c) {
-      double s = sX[i][c];
-      for (int k = 0; k < c; ++k) s -= sX[i][k] * sL[c][k];
-      sX[i][c] = s / sL[c][c];
+    for (int b = 0; b < TB / PB; ++b) {
+      const int b0 = b * PB;
+      double x[PB];
+#pragma unroll
+      for (int k = 0; k < PB; ++k) x[k] = sX[i][b0 + k];
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        x[k] = x[k] / sL[b0 + k][b0 + k];
+#pragma unroll
+        for (int kk = k + 1; kk < PB; ++kk) x[kk] -= x[k] * sL[b0 + kk][b0 + k];
+      }
+#pragma unroll
+      for (int k = 0; k < PB; ++k) sX[i][b0 + k] = x[k];
+      for (int j = b0 + PB; j < TB; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < PB; ++k) acc += x[k] * sL[j][b0 + k];
+        sX[i][j] -= acc;
+      }
     }
   }
   __syncthreads();
