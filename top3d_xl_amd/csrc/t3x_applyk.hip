@@ -183,19 +183,58 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restri
 }
 
 // Coarse-level K*u with the stored Galerkin cell matrices (Solving_KbyU_MatrixFree
-// for iLevel > 1, 727-744): one thread per node (coarse levels are small, so
-// the grid is the parallelism), holding its 27-node neighbourhood in
-// registers and reading, from each of its <= 8 cells, the 3 rows of the
-// cell's 24x24 matrix that belong to it (upper triangle, entry-major: a wave
-// reads one entry of consecutive cells), so every stored entry is read about
-// twice overall.  Coarse levels carry no fixed DOFs (the boundary conditions
-// are inside the Galerkin matrices) and vectors are zero on void nodes.
+// for iLevel > 1, 727-744), in two passes so that every stored entry is read
+// exactly once with fully coalesced loads:
+//   1. one thread per cell forms Y_c = Ks_c u_c (24 values, upper triangle
+//      entry-major: a wave reads one entry of 64 consecutive cells) and
+//      writes Y_c entry-major to a scratch array;
+//   2. one thread per node adds the 3 rows of Y_c that belong to it from
+//      its <= 8 cells in a fixed order and applies the smoother epilogue.
+// The scratch (24 floats per node slot) costs 8% of the matrix bytes.
+// Coarse levels carry no fixed DOFs (the boundary conditions are inside the
+// Galerkin matrices); vectors and matrices are zero on void nodes and cells.
 // FP32 throughout (the 'Standard' V-cycle smoother).
+__global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const float* __restrict__ Ks,
+                                                      const float* __restrict__ u, float* __restrict__ Yc,
+                                                      const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int plane = g.NY * g.NX;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= plane) return;
+  const int cx = idx / g.NY, cy = idx - cx * g.NY;
+  const int cz = blockIdx.y;
+  if (cy >= g.ny || cx >= g.nx || cz >= g.nz) return;
+  const long long NN = g.NN;
+  const long long cell = idx + (long long)plane * cz;
+  float ue[24], ye[24];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const long long m = cell + kLocDy[a] + (long long)g.NY * kLocDx[a] + (long long)plane * kLocDz[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ue[3 * a + c] = u[c * NN + m];
+      ye[3 * a + c] = 0.f;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 24; ++q) {
+#pragma unroll
+    for (int p = 0; p <= q; ++p) {
+      const float v = Ks[(long long)(q * (q + 1) / 2 + p) * NN + cell];
+      ye[p] += v * ue[q];
+      if (p != q) ye[q] += v * ue[p];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 24; ++i) Yc[(long long)i * NN + cell] = ye[i];
+}
+
 template <int EPI>
-__global__ __launch_bounds__(kBlock) void k_apply_Ks_node(Geo g, const float* __restrict__ Ks,
-                                                          const float* __restrict__ u, float* __restrict__ y,
-                                                          const float* __restrict__ rv, const float* __restrict__ dv,
-                                                          double omega, const int* fl) {
+__global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const float* __restrict__ Yc,
+                                                           const float* __restrict__ u, float* __restrict__ y,
+                                                           const float* __restrict__ rv,
+                                                           const float* __restrict__ dv, double omega,
+                                                           const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int plane = g.NY * g.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -204,20 +243,6 @@ __global__ __launch_bounds__(kBlock) void k_apply_Ks_node(Geo g, const float* __
   const int z = blockIdx.y;
   const long long NN = g.NN;
   const long long n = idx + (long long)plane * z;
-  // neighbourhood nb[(ky + 3 kx + 9 kz) * 3 + c] = u(y-1+ky, x-1+kx, z-1+kz)
-  float nb[81];
-#pragma unroll
-  for (int kz = 0; kz < 3; ++kz)
-#pragma unroll
-    for (int kx = 0; kx < 3; ++kx)
-#pragma unroll
-      for (int ky = 0; ky < 3; ++ky) {
-        const int gy = yy - 1 + ky, gx = x - 1 + kx, gz = z - 1 + kz;
-        const bool in = gy >= 0 && gx >= 0 && gz >= 0 && gy < g.NY && gx < g.NX && gz < g.NZ;
-        const long long m = in ? gy + (long long)g.NY * gx + (long long)plane * gz : n;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) nb[(ky + 3 * kx + 9 * kz) * 3 + c] = in ? u[c * NN + m] : 0.f;
-      }
   float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int cc = 0; cc < 8; ++cc) {
@@ -227,15 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_Ks_node(Geo g, const float* __
     const long long cell = cy + (long long)g.NY * cx + (long long)plane * cz;
     const int jl = loc_index(1 - CY, 1 - CX, 1 - CZ);  // this node's local index in the cell
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = (CY + kLocDy[j]) + 3 * (CX + kLocDx[j]) + 9 * (CZ + kLocDz[j]);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float v = nb[k * 3 + c];
-#pragma unroll
-        for (int r = 0; r < 3; ++r) acc[r] += Ks[(long long)tri_index(3 * jl + r, 3 * j + c) * NN + cell] * v;
-      }
-    }
+    for (int r = 0; r < 3; ++r) acc[r] += Yc[(long long)(3 * jl + r) * NN + cell];
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -243,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_Ks_node(Geo g, const float* __
       y[c * NN + n] = rv[c * NN + n] - acc[c];
     } else {
       const float dc = dv[c * NN + n];
-      y[c * NN + n] = dc != 0.f ? nb[13 * 3 + c] + float(omega) * (rv[c * NN + n] - acc[c]) / dc : 0.f;
+      y[c * NN + n] = dc != 0.f ? u[c * NN + n] + float(omega) * (rv[c * NN + n] - acc[c]) / dc : 0.f;
     }
   }
 }
@@ -266,16 +283,20 @@ void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const d
 
 static dim3 node_grid(const Geo& g) { return dim3((g.NY * g.NX + kBlock - 1) / kBlock, g.NZ); }
 
+long long coarse_ks_tmp_len(const Geo& g) { return 24LL * g.NN; }
+
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
-                           const int* fl) {
-  hipLaunchKernelGGL(k_apply_Ks_node<EPI_RESID>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, d, r,
+                           float* tmp, const int* fl) {
+  hipLaunchKernelGGL(k_cell_Ks_u, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
+  hipLaunchKernelGGL(k_node_gather_Ks<EPI_RESID>, node_grid(g), dim3(kBlock), 0, s, g, (const float*)tmp, x, d, r,
                      (const float*)nullptr, 0.0, fl);
 }
 
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
-                            const float* diag, double omega, float* out, const int* fl) {
-  hipLaunchKernelGGL(k_apply_Ks_node<EPI_SMOOTH>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, out, r, diag, omega,
-                     fl);
+                            const float* diag, double omega, float* out, float* tmp, const int* fl) {
+  hipLaunchKernelGGL(k_cell_Ks_u, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
+  hipLaunchKernelGGL(k_node_gather_Ks<EPI_SMOOTH>, node_grid(g), dim3(kBlock), 0, s, g, (const float*)tmp, x, out,
+                     r, diag, omega, fl);
 }
 
 }  // namespace t3x

@@ -170,7 +170,7 @@ struct t3x_problem {
   // cycle adds FP32 Ke/E and the fine-level smoother vectors
   float* z32 = nullptr;
   float *Ke32 = nullptr, *E32 = nullptr, *x32 = nullptr, *d32 = nullptr;
-  void* rtmp = nullptr;  // intermediate of the separable restriction (largest level pair)
+  void* rtmp = nullptr;  // restriction intermediate / coarse smoother products (largest use)
   double *ca = nullptr, *cb = nullptr, *cc = nullptr, *cd = nullptr;  // cell scratch
   // filters (node scalars)
   double *fb = nullptr, *fx = nullptr, *fr = nullptr, *fz = nullptr, *fp = nullptr, *fq = nullptr;
@@ -367,6 +367,7 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
       if (prm.vcycle == 1 && l >= 1 && l < L - 1) add(12 * NN), add(4LL * kTri * NN);  // tf, Ksf
     }
     if (l + 1 < L) tmp = std::max(tmp, 3LL * (d[l + 1].ny + 1) * (d[l].nx + 1) * (d[l].nz + 1));
+    if (prm.vcycle == 1 && l >= 1 && l < L - 1) tmp = std::max(tmp, 12 * NN);  // coarse smoother products
   }
   add(8 * tmp);                                      // restriction intermediate
   const long long NN = NNof(0);
@@ -492,6 +493,8 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   {
     long long tl = 0;
     for (int l = 0; l + 1 < P.L; ++l) tl = std::max(tl, restrict_tmp_len(geo(P.lv[l]), geo(P.lv[l + 1])));
+    if (P.standard())  // also the per-cell products of the coarse smoothers (floats)
+      for (int l = 1; l + 1 < P.L; ++l) tl = std::max(tl, (coarse_ks_tmp_len(geo(P.lv[l])) + 1) / 2);
     P.rtmp = P.alloc<double>(tl);
   }
   P.fix = P.alloc<uint8_t>(NN);
@@ -710,7 +713,9 @@ void vcycle_mixed(t3x_problem& P, const double* r, float* z, double* sc, const i
 // on the way up.  Fine-level K*u with E*Ke and the BC rows, coarse levels
 // with the FP32 copies of the Galerkin matrices; z.r fused into the last
 // smoothing step.
-void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, const int* fl, bool fuse) {
+// x0Ready: the fine x = w r/d was already formed (by the PCG update kernel).
+void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, const int* fl, bool fuse,
+                     bool x0Ready = false) {
   hipStream_t s = P.stream;
   const int L = P.L;
   const double om = P.P.omega;
@@ -718,13 +723,13 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   const Geo g0 = geo(P.lv[0]);
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
-  launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl);                      // 757
+  if (!x0Ready) launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl);        // 757
   launch_fine_resid32_mfma(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.d32, fl);              // 761
   launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const float*)P.d32, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
     launch_jacobi0(s, (const float*)V.rf, V.diagf, V.xf, om, 3 * V.NN, fl);
-    launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, fl);
+    launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, tmp, fl);
     launch_restrict(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const float*)V.tf, tmp, P.lv[l + 1].nmask,
                     P.lv[l + 1].rf, fl);
   }
@@ -735,7 +740,7 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   for (int l = L - 2; l >= 1; --l) {
     Level& V = P.lv[l];
     launch_prolong_add(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, P.lv[l + 1].xf, V.xf, V.nmask, fl);  // 770
-    launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, fl);          // 775
+    launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, tmp, fl);     // 775
     std::swap(V.xf, V.tf);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, P.lv[1].xf, P.x32, P.lv[0].nmask, fl);
@@ -774,9 +779,12 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
     P.tbegin(TG_APPLYK, ek);
     launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                 // 614-615
     P.tend(TG_APPLYK, ek);
-    launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                               // 617-622
+    if (P.standard())  // 617-622, with the first fine Jacobi step of 628-629 fused
+      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl, P.lv[0].diagf, P.x32, P.P.omega);
+    else
+      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                             // 617-622
     if (mx) {
-      if (P.standard()) vcycle_standard(P, P.r, P.z32, sc, fl, true);                      // 628-629
+      if (P.standard()) vcycle_standard(P, P.r, P.z32, sc, fl, true, true);                // 628-629
       else vcycle_mixed(P, P.r, P.z32, sc, fl, true);
       launch_pcg_pupdate(s, y, P.p, P.z32, n3, sc, fl);                                     // 616, 630
     } else {

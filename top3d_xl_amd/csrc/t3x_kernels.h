@@ -89,10 +89,12 @@ void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, dou
                     const int* fl);
 // FP32 K*u smoothers of coarse levels with the stored Galerkin matrices (FP32
 // copy; t3x_applyk.hip).  resid: d = r - Kx; smooth: out = x + omega (r - Kx) / diag.
+// tmp: coarse_ks_tmp_len(g) floats of scratch (per-cell products).
+long long coarse_ks_tmp_len(const Geo& g);
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
-                           const int* fl);
+                           float* tmp, const int* fl);
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
-                            const float* diag, double omega, float* out, const int* fl);
+                            const float* diag, double omega, float* out, float* tmp, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const float* r, double* v, const int* fl);
 void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long NN3,
@@ -112,8 +114,11 @@ void launch_copy(hipStream_t s, double* dst, const double* src, long long n);
 void launch_sub(hipStream_t s, double* r, const double* b, const double* q, long long n);  // r = b - q
 void launch_norm2(hipStream_t s, const double* v, long long n, const RedWs& ws, double* out);
 void launch_dot(hipStream_t s, const double* a, const double* b, long long n, const RedWs& ws, double* out);
+// r -= lambda q with r.r; with xj, also xj = omega r / dj (first Jacobi step of
+// the FP32 'Standard' V-cycle).  y and p are updated later by launch_pcg_pupdate.
 void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, const double* q, long long n,
-                       const RedWs& ws, double* sc, int* fl);
+                       const RedWs& ws, double* sc, int* fl, const float* dj = nullptr, float* xj = nullptr,
+                       double omega = 0.0);
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
                         const int* fl);
 void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl);

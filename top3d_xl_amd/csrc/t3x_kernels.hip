@@ -699,9 +699,13 @@ void launch_dot(hipStream_t s, const double* a, const double* b, long long n, co
 }
 
 // y += lambda p ; r -= lambda q ; resnorm = ||r|| / ||b|| ; convergence flag.
-__global__ __launch_bounds__(kBlock) void k_pcg_update(double* __restrict__ y, double* __restrict__ r,
-                                                       const double* __restrict__ p, const double* __restrict__ q,
-                                                       long long n, RedWs ws, double* sc, int* fl) {
+// r -= lambda q (617) with the r.r reduction.  With xj given (FP32 'Standard'
+// V-cycle), also the first fine Jacobi step of the next preconditioner
+// application, xj = omega r / dj (757), while r is in registers.
+__global__ __launch_bounds__(kBlock) void k_pcg_update(double* __restrict__ r, const double* __restrict__ q,
+                                                       long long n, RedWs ws, double* sc, int* fl,
+                                                       const float* __restrict__ dj, float* __restrict__ xj,
+                                                       double omega) {
   if (fl[FL_DONE]) return;
   // y += lambda p (616) is deferred to k_pcg_pupdate, which reads p anyway
   const double lambda = sc[SC_LAMBDA];
@@ -710,15 +714,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(double* __restrict__ y, d
     const double ri = r[i] - lambda * q[i];     // 617
     r[i] = ri;
     rr += ri * ri;
+    if (xj) {
+      const double dd = (double)dj[i];
+      xj[i] = dd != 0.0 ? (float)(omega * (ri / dd)) : 0.f;
+    }
   }
-  (void)y;
-  (void)p;
   double v[1] = {rr};
   write_partials<1>(v, ws.partials);
 }
 void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, const double* q, long long n,
-                       const RedWs& ws, double* sc, int* fl) {
-  hipLaunchKernelGGL(k_pcg_update, dim3(red_blocks(n)), dim3(kBlock), 0, s, y, r, p, q, n, ws, sc, fl);
+                       const RedWs& ws, double* sc, int* fl, const float* dj, float* xj, double omega) {
+  (void)y;
+  (void)p;
+  hipLaunchKernelGGL(k_pcg_update, dim3(red_blocks(n)), dim3(kBlock), 0, s, r, q, n, ws, sc, fl, dj, xj, omega);
   fin1(s, FIN_RR, red_blocks(n), ws, sc, fl, nullptr);
 }
 
