@@ -16,10 +16,12 @@ struct CoarseSolver {
   void plan(int n, int bw);
   size_t band_len() const;
   size_t tinv_len() const { return (size_t)np * 64; }
+  // two panels of row tiles (64x64) below a diagonal tile: the factorization's scratch
+  size_t panel_len() const { return 2 * (size_t)((bw + 63) / 64 + 1) * 64 * 64; }
   size_t blk_len() const { return (size_t)nblk * SB * SB; }
   void pad_identity(hipStream_t s, double* band);
-  void factor(hipStream_t s, double* band, double* tinv, double* Dinv, double* DT, double* Gm, double* H,
-              int* status);
+  void factor(hipStream_t s, double* band, double* tinv, double* pan, double* Dinv, double* DT, double* Gm,
+              double* H, int* status);
   // FP64 solve; the right-hand side is gathered from / the solution scattered
   // to dense level vectors of either precision
   void solve(hipStream_t s, const long long* map, const double* rdense, double* xdense, const double* Dinv,

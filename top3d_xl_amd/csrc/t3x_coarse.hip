@@ -4,10 +4,16 @@
 // The coarsest Galerkin matrix is banded after an axis-permuted node order
 // (bandwidth bw).  The reference factors it with CHOLMOD; here:
 //
-//  1. right-looking blocked banded Cholesky, 64-column panels, two launches
-//     per panel (TRSM of the panel, SYRK of the trailing band).  The work-
-//     group that finishes the update of the next diagonal tile factors it and
-//     stores the tile's inverse, so no serial phase sits between launches.
+//  1. right-looking blocked banded Cholesky, 64-column panels, one launch per
+//     panel: every work-group of the trailing update solves the panel tiles
+//     it needs (X = A L_JJ^{-T}, blocked by 16 columns on the matrix cores
+//     with the inverses of L_JJ's 16x16 diagonal blocks) and applies its
+//     X_I X_K^T update on the matrix cores; the group owning the next
+//     diagonal tile then factors it (register/readlane POTRF), so the only
+//     serial chain per panel is that tile's pivots.  Solved panel tiles go
+//     to a scratch panel and reach the band in the next launch (the band's
+//     A(I, J) is still being read during the current one).  The 64x64 tile
+//     inverses for the solves are formed afterwards, in parallel.
 //  2. k_trsm_blocks / k_h_blocks: the band is cut into solve blocks of SB >=
 //     bw rows, so each block couples only to its predecessor:
 //        forward   y_b = Dinv_b v_b - G_b y_{b-1},   G_b = Dinv_b L_{b,b-1}
@@ -54,100 +60,233 @@ struct BandView {
 };
 
 // ---- 64x64 tile helpers in LDS -----------------------------------------
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
 // In-place Cholesky of the lower triangle of sA, blocked in 4 panels of 16
-// columns: the block's first wave factors a panel in registers (lane i owns
-// row i; the pivot and the column entries travel by cross-lane shuffles, no
-// LDS round-trip per element), then all threads apply the panel's rank-16
-// update to the trailing lower triangle.
-__device__ void tile_potrf(double (*sA)[TB + 1], int* status) {
+// columns.  The block's first wave factors a 64x16 panel in registers (lane i
+// owns row i): per column the pivot and the column entries are broadcast with
+// readlane (compile-time lanes), the pivot's reciprocal square root scales
+// the column, and the rank-1 update stays in registers.  Every lane applies
+// the same operations branch-free: a lane's entries right of its diagonal
+// only ever hold values that are neither broadcast nor stored.  All threads
+// then apply the panel's rank-16 update to the trailing triangle in 4x4
+// register blocks.
+__device__ __forceinline__ void tile_potrf(double (*sA)[TB + 1], int* status) {
   constexpr int PB = 16;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
   for (int b = 0; b < TB / PB; ++b) {
     const int b0 = b * PB;
-    if (threadIdx.x < TB) {
-      const int i = threadIdx.x;
+    if (threadIdx.x < 64) {
       double p[PB];
 #pragma unroll
-      for (int k = 0; k < PB; ++k) p[k] = sA[i][b0 + k];
+      for (int k = 0; k < PB; ++k) p[k] = sA[lane][b0 + k];
       bool bad = false;
 #pragma unroll
       for (int k = 0; k < PB; ++k) {
-        const int c = b0 + k;
-        const double piv = __shfl(p[k], c, 64);
-        const double d = sqrt(piv);
+        const double piv = readlane_d(p[k], b0 + k);
         bad |= !(piv > 0.0);
-        if (i == c) p[k] = d;
-        else if (i > c) p[k] = p[k] / d;
+        p[k] *= rsqrt(piv);  // the pivot's lane gets sqrt(piv), lanes below L(i, c)
 #pragma unroll
-        for (int kk = k + 1; kk < PB; ++kk) {
-          const double ljc = __shfl(p[k], b0 + kk, 64);  // L(b0 + kk, c)
-          if (i >= b0 + kk) p[kk] -= p[k] * ljc;
-        }
+        for (int kk = k + 1; kk < PB; ++kk) p[kk] -= p[k] * readlane_d(p[k], b0 + kk);
       }
-      if (i >= b0) {
+      if (lane >= b0) {
 #pragma unroll
         for (int k = 0; k < PB; ++k)
-          if (b0 + k <= i) sA[i][b0 + k] = p[k];
+          if (b0 + k <= lane) sA[lane][b0 + k] = p[k];
       }
-      if (bad && i == b0) *status = 1;
+      if (bad && lane == 0) *status = 1;
     }
     __syncthreads();
-    // trailing update: rows and columns >= b0 + PB, lower triangle
-    const int t0 = b0 + PB, m = TB - t0;
-    for (int t = threadIdx.x; t < m * m; t += CT) {
-      const int i = t0 + t / m, j = t0 + t % m;
-      if (j > i) continue;
-      double acc = 0.0;
+    if (b0 + PB < TB) {  // trailing lower triangle: rows and columns >= b0 + PB
+      const int t0 = b0 + PB, mb = (TB - t0) / 4;
+      for (int t = threadIdx.x; t < mb * mb; t += CT) {
+        const int bi = t / mb, bj = t % mb;
+        if (bj > bi) continue;
+        const int i0 = t0 + 4 * bi, j0 = t0 + 4 * bj;
+        double acc[4][4] = {};
 #pragma unroll
-      for (int k = 0; k < PB; ++k) acc += sA[i][b0 + k] * sA[j][b0 + k];
-      sA[i][j] -= acc;
+        for (int k = 0; k < PB; ++k) {
+          double a[4], c4[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            a[r] = sA[i0 + r][b0 + k];
+            c4[r] = sA[j0 + r][b0 + k];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[r][q] += a[r] * c4[q];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (j0 + q <= i0 + r) sA[i0 + r][j0 + q] -= acc[r][q];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
-// Inverse of the lower-triangular sA into sI (one column per thread).
-__device__ void tile_trinv(const double (*sA)[TB + 1], double (*sI)[TB + 1]) {
-  if (threadIdx.x < TB) {
-    const int c = threadIdx.x;
-    for (int i = 0; i < c; ++i) sI[i][c] = 0.0;
-    sI[c][c] = 1.0 / sA[c][c];
-    for (int i = c + 1; i < TB; ++i) {
-      double s = 0.0;
-      for (int k = c; k < i; ++k) s += sA[i][k] * sI[k][c];
-      sI[i][c] = -s / sA[i][i];
+// Inverse of the lower-triangular sA (64x64) into sI, blocked in 16x16:
+// wave w inverts diagonal block w (lane = column, forward substitution with
+// reciprocal diagonals), then the off-diagonal blocks follow by block
+// diagonals d = 1, 2, 3:  X_IJ = -X_II (sum_{J<=K<I} L_IK X_KJ), I = J + d.
+__device__ __forceinline__ void tile_trinv(const double (*sA)[TB + 1], double (*sI)[TB + 1], double (*sT)[TB + 1]) {
+  constexpr int PB = 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < TB * TB; t += CT) sI[t / TB][t % TB] = 0.0;
+  __syncthreads();
+  if (lane < PB) {
+    const int o = wave * PB, c = lane;
+    double x[PB];
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= sA[o + i][o + k] * x[k];
+      x[i] = (i >= c) ? s / sA[o + i][o + i] : 0.0;
     }
+#pragma unroll
+    for (int i = 0; i < PB; ++i) sI[o + i][o + c] = x[i];
   }
   __syncthreads();
+  for (int d = 1; d < TB / PB; ++d) {
+    const int nb = TB / PB - d;  // blocks (J + d, J), J = 0 .. nb-1
+    for (int t = threadIdx.x; t < nb * PB * PB; t += CT) {  // T = sum_K L_IK X_KJ
+      const int J = t / (PB * PB), a = (t / PB) % PB, c = t % PB, I = J + d;
+      double s = 0.0;
+      for (int k = J * PB; k < I * PB; ++k) s += sA[I * PB + a][k] * sI[k][J * PB + c];
+      sT[I * PB + a][J * PB + c] = s;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nb * PB * PB; t += CT) {  // X_IJ = -X_II T
+      const int J = t / (PB * PB), a = (t / PB) % PB, c = t % PB, I = J + d;
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < PB; ++k) s += sI[I * PB + a][I * PB + k] * sT[I * PB + k][J * PB + c];
+      sI[I * PB + a][J * PB + c] = -s;
+    }
+    __syncthreads();
+  }
 }
 
-// acc[4][4] += A(4ti.., :) * B(:, 4tj..) for thread (ti, tj) = (t/16, t%16);
-// A given row-major sA[m][k], B given as sB[k][n] (transB = false) or
-// sB[n][k] (transB = true).
-template <bool transB>
-__device__ __forceinline__ void tile_gemm_acc(const double (*sA)[TB + 1], const double (*sB)[TB + 1],
-                                              double (&acc)[4][4]) {
-  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
-#pragma unroll 4
-  for (int k = 0; k < TB; ++k) {
-    double a[4], b[4];
+
+typedef double cd4 __attribute__((ext_vector_type(4)));
+// acc = A B^T on v_mfma_f64_16x16x4 (A, B: 64x64 row-major in LDS); wave w
+// owns the 32x32 quarter (32 (w / 2), 32 (w % 2)) as 2x2 tiles of 16x16.
+// D[r][c] of a tile at lane c + 16 (r % 4), register r / 4.
+__device__ __forceinline__ void tile_gemm_nt_mfma(const double (*sA)[TB + 1], const double (*sB)[TB + 1],
+                                                  cd4 (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, kq = lane >> 4;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[r] = sA[4 * ti + r][k];
+  for (int kb = 0; kb < TB / 4; ++kb) {
+    const double a0 = sA[r0 + i][4 * kb + kq], a1 = sA[r0 + 16 + i][4 * kb + kq];
+    const double b0 = sB[c0 + i][4 * kb + kq], b1 = sB[c0 + 16 + i][4 * kb + kq];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+// (row, col) in the 64x64 product of element (a, b, r4) of tile_gemm_nt_mfma's acc
+__device__ __forceinline__ int nt_row(int a, int r4) {
+  return 32 * ((threadIdx.x >> 6) >> 1) + 16 * a + ((threadIdx.x & 63) >> 4) + 4 * r4;
+}
+__device__ __forceinline__ int nt_col(int b) { return 32 * ((threadIdx.x >> 6) & 1) + 16 * b + (threadIdx.x & 15); }
+// acc += A B (A, B: 64x64 row-major in LDS), same output layout as above.
+__device__ __forceinline__ void tile_gemm_nn_mfma(const double (*sA)[TB + 1], const double (*sB)[TB + 1],
+                                                  cd4 (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, kq = lane >> 4;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) b[q] = transB ? sB[4 * tj + q][k] : sB[k][4 * tj + q];
+  for (int kb = 0; kb < TB / 4; ++kb) {
+    const double a0 = sA[r0 + i][4 * kb + kq], a1 = sA[r0 + 16 + i][4 * kb + kq];
+    const double b0 = sB[4 * kb + kq][c0 + i], b1 = sB[4 * kb + kq][c0 + 16 + i];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// Inverses of the four 16x16 diagonal blocks of the lower-triangular sL into
+// sD[4][16][16]: wave w inverts block w, lane c (< 16) one column, by
+// forward substitution with independent updates after each step; the
+// reciprocal diagonal is formed first (lane k), off the dependency chain.
+__device__ __forceinline__ void diag16_inverses(const double (*sL)[TB + 1], double (*sD)[16][17]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, o = 16 * w;
+  const double rdl = lane < 16 ? 1.0 / sL[o + lane][o + lane] : 0.0;
+  if (lane < 16) {
+    const int c = lane;
+    double bvec[16];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int i = 0; i < 16; ++i) bvec[i] = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[r][q] += a[r] * b[q];
+    for (int k = 0; k < 16; ++k) {
+      const double xk = bvec[k] * readlane_d(rdl, k);  // zero above the column's diagonal
+      bvec[k] = xk;
+#pragma unroll
+      for (int i = k + 1; i < 16; ++i) bvec[i] -= sL[o + i][o + k] * xk;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) sD[w][i][c] = bvec[i];
+  }
+}
+
+// X = A L^{-T} in place (sX: A on entry, X on exit), blocked by 16 columns on
+// the matrix cores:  R_b = A_b - X_{<b} L_{b,<b}^T,  X_b = R_b Dinv_b^T,
+// Dinv_b the inverses of L's diagonal blocks.  Wave w owns rows 16w..16w+15
+// throughout, so the waves need no barrier between the steps.
+__device__ __forceinline__ void tile_trsm_rt(double (*sX)[TB + 1], const double (*sL)[TB + 1],
+                                             const double (*sD)[16][17]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, i = lane & 15, kq = lane >> 4;
+  const int r0 = 16 * w;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    cd4 acc;
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) acc[r4] = sX[r0 + kq + 4 * r4][16 * b + i];  // D layout: row kq + 4 r4, col i
+#pragma unroll
+    for (int kb = 0; kb < 4 * b; ++kb)  // -= X_{<b} L_{b,<b}^T
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-sX[r0 + i][4 * kb + kq], sL[16 * b + i][4 * kb + kq], acc, 0, 0, 0);
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) sX[r0 + kq + 4 * r4][16 * b + i] = acc[r4];  // R_b
+    cd4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)  // X_b = R_b Dinv_b^T
+      x = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[r0 + i][16 * b + 4 * kb + kq], sD[b][i][4 * kb + kq], x, 0, 0, 0);
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) sX[r0 + kq + 4 * r4][16 * b + i] = x[r4];
+  }
+}
+
+// Stages a 64x64 tile of the band (rows i0.., columns k0..) into LDS: all 16
+// loads of a thread are issued before the stores.
+__device__ __forceinline__ void stage_tile(BandView V, int i0, int k0, double (*sT)[TB + 1]) {
+  double v[TB * TB / CT];
+#pragma unroll
+  for (int u = 0; u < TB * TB / CT; ++u) {
+    const int t = threadIdx.x + u * CT;
+    v[u] = V.get(i0 + t / TB, k0 + t % TB);
+  }
+#pragma unroll
+  for (int u = 0; u < TB * TB / CT; ++u) {
+    const int t = threadIdx.x + u * CT;
+    sT[t / TB][t % TB] = v[u];
   }
 }
 
 // ---- 1. banded Cholesky ---------------------------------------------------
-// Factor diagonal tile d0 (global row/col) in place.
-__device__ void diag_tile(BandView V, int d0, double (*sA)[TB + 1], int* status) {
-  for (int t = threadIdx.x; t < TB * TB; t += CT) {
-    const int i = t / TB, k = t % TB;
-    sA[i][k] = V.get(d0 + i, d0 + k);
-  }
+__global__ __launch_bounds__(CT) void k_chol_diag(BandView V, int d0, int* status) {
+  __shared__ double sA[TB][TB + 1];
+  for (int t = threadIdx.x; t < TB * TB; t += CT) sA[t / TB][t % TB] = V.get(d0 + t / TB, d0 + t % TB);
   __syncthreads();
   tile_potrf(sA, status);
   for (int t = threadIdx.x; t < TB * TB; t += CT) {
@@ -156,98 +295,92 @@ __device__ void diag_tile(BandView V, int d0, double (*sA)[TB + 1], int* status)
   }
 }
 
-__global__ __launch_bounds__(CT) void k_chol_diag(BandView V, int d0, int* status) {
-  __shared__ double sA[TB][TB + 1];
-  diag_tile(V, d0, sA, status);
-}
-
-// L(i, J) = A(i, J) * L_JJ^{-T} for the row tiles below diagonal tile J, by
-// forward substitution in 4 column blocks of 16: lane i solves its row's
-// block in registers (right-looking: independent FMAs after each division;
-// entries of L_JJ are uniform LDS broadcasts), then updates the row's later
-// columns in LDS with the block (a rank-16 update).
-__global__ __launch_bounds__(CT) void k_chol_trsm(BandView V, int J) {
-  constexpr int PB = 16;
-  __shared__ double sL[TB][TB + 1];
-  __shared__ double sX[TB][TB + 1];
-  const int j0 = J * TB, r0 = j0 + TB;
-  const int i0 = r0 + blockIdx.x * TB;
-  for (int t = threadIdx.x; t < TB * TB; t += CT) {
-    const int i = t / TB, c = t % TB;
-    sL[i][c] = V.get(j0 + i, j0 + c);
-    sX[i][c] = V.get(i0 + i, j0 + c);
-  }
-  __syncthreads();
-  if (threadIdx.x < TB) {
-    const int i = threadIdx.x;
-    for (int b = 0; b < TB / PB; ++b) {
-      const int b0 = b * PB;
-      double x[PB];
-#pragma unroll
-      for (int k = 0; k < PB; ++k) x[k] = sX[i][b0 + k];
-#pragma unroll
-      for (int k = 0; k < PB; ++k) {
-        x[k] = x[k] / sL[b0 + k][b0 + k];
-#pragma unroll
-        for (int kk = k + 1; kk < PB; ++kk) x[kk] -= x[k] * sL[b0 + kk][b0 + k];
-      }
-#pragma unroll
-      for (int k = 0; k < PB; ++k) sX[i][b0 + k] = x[k];
-      for (int j = b0 + PB; j < TB; ++j) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < PB; ++k) acc += x[k] * sL[j][b0 + k];
-        sX[i][j] -= acc;
-      }
-    }
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < TB * TB; t += CT) V.set(i0 + t / TB, j0 + t % TB, sX[t / TB][t % TB]);
-}
-
-// Trailing update of the band by panel J; tile 0 is the next diagonal tile,
-// which its work-group factors right after updating it.
-__global__ __launch_bounds__(CT) void k_chol_syrk(BandView V, int J, int* status) {
-  __shared__ double sA[TB][TB + 1];
-  __shared__ double sB[TB][TB + 1];
+// One launch per 64-column panel J (right-looking, the panel solve folded into
+// the trailing update).  Work-group (I, K), K <= I, over the row tiles below
+// diagonal tile J that the band reaches:
+//   X_I = A(I, J) L_JJ^{-T}, X_K = A(K, J) L_JJ^{-T}   (substitution in LDS)
+//   A(I, K) -= X_I X_K^T                               (matrix cores)
+// The (I, I) groups keep L(I, J) = X_I in a scratch panel (the band's copy of
+// A(I, J) is still being read by the other groups); the next launch's extra
+// groups copy it into the band.  Group (0, 0) updates the next diagonal tile
+// and factors it.
+__global__ __launch_bounds__(CT) void k_chol_panel(BandView V, int J, int nrt, int nrtPrev,
+                                                   double* __restrict__ pan, int panStride, int* status) {
+  __shared__ double sL[TB][TB + 1];  // L_JJ (zeros above the diagonal)
+  __shared__ double sA[TB][TB + 1];  // A(I, J) -> X_I
+  __shared__ double sB[TB][TB + 1];  // A(K, J) -> X_K, or the next diagonal tile
+  __shared__ double sD[4][16][17];   // inverses of L_JJ's 16x16 diagonal blocks
   const int bw = V.bw, n = V.n;
   const int j0 = J * TB, r0 = j0 + TB;
+  const int npairs = nrt * (nrt + 1) / 2;
+  if ((int)blockIdx.x >= npairs) {  // copy row tile t of panel J - 1 into the band
+    const int t = blockIdx.x - npairs;
+    if (t >= nrtPrev) return;
+    const double* src = pan + (size_t)((J - 1) & 1) * panStride + (size_t)t * TB * TB;
+    const int i0 = j0 + t * TB, k0 = j0 - TB;
+    for (int e = threadIdx.x; e < TB * TB; e += CT) V.set(i0 + e / TB, k0 + e % TB, src[e]);
+    return;
+  }
   const int t0 = blockIdx.x;
   int I = 0;
   while ((I + 1) * (I + 2) / 2 <= t0) ++I;
   const int K = t0 - I * (I + 1) / 2;
   const int i0 = r0 + I * TB, k0 = r0 + K * TB;
   if (i0 - (k0 + TB - 1) > bw) return;  // tile outside the band
-  for (int t = threadIdx.x; t < TB * TB; t += CT) {
-    const int a = t / TB, c = t % TB;
-    sA[a][c] = V.get(i0 + a, j0 + c);
-    sB[a][c] = V.get(k0 + a, j0 + c);
-  }
+  stage_tile(V, j0, j0, sL);  // zeros above the diagonal
+  stage_tile(V, i0, j0, sA);
+  if (K != I) stage_tile(V, k0, j0, sB);
   __syncthreads();
-  double acc[4][4] = {};
-  tile_gemm_acc<true>(sA, sB, acc);  // A(i, J) * A(k, J)^T
-  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = i0 + 4 * ti + r, k = k0 + 4 * tj + q;
-      if (k <= i && i - k <= bw && i < n) V.B[(size_t)i * V.W() + (i - k)] -= acc[r][q];
-    }
-  if (t0 == 0) {
-    __syncthreads();
-    diag_tile(V, r0, sA, status);
+  diag16_inverses(sL, sD);
+  __syncthreads();
+  tile_trsm_rt(sA, sL, sD);
+  if (K != I) tile_trsm_rt(sB, sL, sD);
+  __syncthreads();
+  if (K == I) {  // L(I, J) to the scratch panel
+    double* dst = pan + (size_t)(J & 1) * panStride + (size_t)I * TB * TB;
+    for (int t = threadIdx.x; t < TB * TB; t += CT) dst[t] = sA[t / TB][t % TB];
   }
+  cd4 acc[2][2] = {};
+  tile_gemm_nt_mfma(sA, K == I ? sA : sB, acc);  // X_I X_K^T
+  if (I == 0) {  // the next diagonal tile: update in LDS, then factor
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+          const int rr = nt_row(a, r4), cc = nt_col(b);
+          sB[rr][cc] = V.get(r0 + rr, r0 + cc) - acc[a][b][r4];
+        }
+    __syncthreads();
+    tile_potrf(sB, status);
+    for (int t = threadIdx.x; t < TB * TB; t += CT) {
+      const int i = t / TB, k = t % TB;
+      if (k <= i) V.set(r0 + i, r0 + k, sB[i][k]);
+    }
+    return;
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4) {
+        const int i = i0 + nt_row(a, r4), k = k0 + nt_col(b);
+        if (k <= i && i - k <= bw && i < n) V.B[(size_t)i * V.W() + (i - k)] -= acc[a][b][r4];
+      }
 }
 
 // Inverses of all 64x64 diagonal tiles (after the factorization, in parallel).
 __global__ __launch_bounds__(CT) void k_tile_inverses(BandView V, double* __restrict__ tinv) {
   __shared__ double sA[TB][TB + 1];
   __shared__ double sI[TB][TB + 1];
+  __shared__ double sT[TB][TB + 1];
   const int d0 = blockIdx.x * TB;
   for (int t = threadIdx.x; t < TB * TB; t += CT) sA[t / TB][t % TB] = V.get(d0 + t / TB, d0 + t % TB);
   __syncthreads();
-  tile_trinv(sA, sI);
+  tile_trinv(sA, sI, sT);
   double* inv = tinv + (size_t)blockIdx.x * TB * TB;
   for (int t = threadIdx.x; t < TB * TB; t += CT) inv[t] = sI[t / TB][t % TB];
 }
@@ -287,42 +420,38 @@ __global__ __launch_bounds__(CT) void k_trsm_blocks(BandView V, int SB, const do
       sT[a][c] = r;
     }
     __syncthreads();
-    double acc[4][4] = {};
+    cd4 acc[2][2] = {};
     for (int K = Ist; K < I; ++K) {
       if ((I * TB) - (K * TB + TB - 1) > V.bw) continue;  // L tile outside the band
-      for (int t = threadIdx.x; t < TB * TB; t += CT) {
-        const int a = t / TB, c = t % TB;
-        sL[a][c] = V.get(base + I * TB + a, base + K * TB + c);
-        sX[a][c] = Out[(size_t)(K * TB + a) * SB + col0 + c];
-      }
+      stage_tile(V, base + I * TB, base + K * TB, sL);
+      for (int t = threadIdx.x; t < TB * TB; t += CT) sX[t / TB][t % TB] = Out[(size_t)(K * TB + t / TB) * SB + col0 + t % TB];
       __syncthreads();
-      tile_gemm_acc<false>(sL, sX, acc);
+      tile_gemm_nn_mfma(sL, sX, acc);
       __syncthreads();
     }
-    {
-      const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sT[4 * ti + r][4 * tj + q] -= acc[r][q];
-    }
-    __syncthreads();
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) sT[nt_row(a, r4)][nt_col(c)] -= acc[a][c][r4];
     // X_I = inv(L_II) T
     const double* inv = tinv + (size_t)(base / TB + I) * TB * TB;
     for (int t = threadIdx.x; t < TB * TB; t += CT) sL[t / TB][t % TB] = inv[t];
     __syncthreads();
     {
-      double acc2[4][4] = {};
-      tile_gemm_acc<false>(sL, sT, acc2);  // inv(L_II) is lower triangular: the zeros add nothing
-      const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+      cd4 acc2[2][2] = {};
+      tile_gemm_nn_mfma(sL, sT, acc2);  // inv(L_II) is lower triangular: the zeros add nothing
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int a2 = 0; a2 < 2; ++a2)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int a = 4 * ti + r, c = 4 * tj + q;
-          Out[(size_t)(I * TB + a) * SB + col0 + c] = acc2[r][q];
-          if (ident) DT[(size_t)b * SB * SB + (size_t)(col0 + c) * SB + I * TB + a] = acc2[r][q];
-        }
+        for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+          for (int r4 = 0; r4 < 4; ++r4) {
+            const int a = nt_row(a2, r4), c = nt_col(c2);
+            Out[(size_t)(I * TB + a) * SB + col0 + c] = acc2[a2][c2][r4];
+            if (ident) DT[(size_t)b * SB * SB + (size_t)(col0 + c) * SB + I * TB + a] = acc2[a2][c2][r4];
+          }
     }
     __syncthreads();
   }
@@ -340,25 +469,23 @@ __global__ __launch_bounds__(CT) void k_h_blocks(BandView V, int SB, int nblk, c
   const int I = rem / st, J = rem % st;
   if (b >= nblk - 1) return;
   const double* D = DT + (size_t)b * SB * SB;
-  double acc[4][4] = {};
+  cd4 acc[2][2] = {};
   for (int K = I; K < st; ++K) {  // DT_b(i,k) = Dinv_b(k,i) is zero for k < i
     const int rowL0 = (b + 1) * SB + J * TB, colL0 = b * SB + K * TB;
     if (rowL0 - (colL0 + TB - 1) > V.bw) continue;
-    for (int t = threadIdx.x; t < TB * TB; t += CT) {
-      const int a = t / TB, c = t % TB;
-      sA[a][c] = D[(size_t)(I * TB + a) * SB + K * TB + c];
-      sB[a][c] = V.get(rowL0 + a, colL0 + c);  // row j, col k
-    }
+    for (int t = threadIdx.x; t < TB * TB; t += CT) sA[t / TB][t % TB] = D[(size_t)(I * TB + t / TB) * SB + K * TB + t % TB];
+    stage_tile(V, rowL0, colL0, sB);  // row j, col k
     __syncthreads();
-    tile_gemm_acc<true>(sA, sB, acc);
+    tile_gemm_nt_mfma(sA, sB, acc);
     __syncthreads();
   }
-  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      H[(size_t)b * SB * SB + (size_t)(I * TB + 4 * ti + r) * SB + J * TB + 4 * tj + q] = acc[r][q];
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r4 = 0; r4 < 4; ++r4)
+        H[(size_t)b * SB * SB + (size_t)(I * TB + nt_row(a, r4)) * SB + J * TB + nt_col(c)] = acc[a][c][r4];
 }
 
 // ---- 3. the solve --------------------------------------------------------
@@ -439,22 +566,25 @@ void CoarseSolver::plan(int n_, int bw_) {
 
 size_t CoarseSolver::band_len() const { return (size_t)np * (bw + 1); }
 
-void CoarseSolver::factor(hipStream_t s, double* band, double* tinv, double* Dinv, double* DT, double* Gm,
-                          double* H, int* status) {
+void CoarseSolver::factor(hipStream_t s, double* band, double* tinv, double* pan, double* Dinv, double* DT,
+                          double* Gm, double* H, int* status) {
   BandView V{band, np, bw};
   const int nt = np / TB;
+  const int panStride = (int)(panel_len() / 2);
   hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, 0, status);
+  int nrtPrev = 0;
   for (int J = 0; J < nt; ++J) {
     const int j0 = J * TB, r0 = j0 + TB;
     const int rlast = std::min(np - 1, j0 + TB - 1 + bw);
     const int nrt = rlast >= r0 ? (rlast - r0) / TB + 1 : 0;
-    if (nrt > 0) {
-      hipLaunchKernelGGL(k_chol_trsm, dim3(nrt), dim3(CT), 0, s, V, J);
-      hipLaunchKernelGGL(k_chol_syrk, dim3(nrt * (nrt + 1) / 2), dim3(CT), 0, s, V, J, status);
-    } else if (J + 1 < nt) {
-      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, r0, status);
-    }
+    const int grid = nrt * (nrt + 1) / 2 + nrtPrev;
+    if (nrt > 0 || nrtPrev > 0)
+      hipLaunchKernelGGL(k_chol_panel, dim3(grid), dim3(CT), 0, s, V, J, nrt, nrtPrev, pan, panStride, status);
+    if (nrt == 0 && J + 1 < nt) hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(CT), 0, s, V, r0, status);
+    nrtPrev = nrt;
   }
+  if (nrtPrev > 0)  // copy the last panel
+    hipLaunchKernelGGL(k_chol_panel, dim3(nrtPrev), dim3(CT), 0, s, V, nt, 0, nrtPrev, pan, panStride, status);
   hipLaunchKernelGGL(k_tile_inverses, dim3(nt), dim3(CT), 0, s, V, tinv);
   const int st = SB / TB;
   hipLaunchKernelGGL(k_trsm_blocks, dim3(nblk * 2 * st), dim3(CT), 0, s, V, SB, tinv, Dinv, DT, Gm);

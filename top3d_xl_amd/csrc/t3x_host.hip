@@ -185,7 +185,7 @@ struct t3x_problem {
   CoarseSolver cs;
   long long* dCMap = nullptr;
   int* dNodePos = nullptr;
-  double *band = nullptr, *tinv = nullptr, *cDinv = nullptr, *cDT = nullptr, *cG = nullptr, *cH = nullptr,
+  double *band = nullptr, *tinv = nullptr, *cPan = nullptr, *cDinv = nullptr, *cDT = nullptr, *cG = nullptr, *cH = nullptr,
          *cva = nullptr, *cvb = nullptr;
   int* dStatus = nullptr;
   bool assembled = false;
@@ -394,6 +394,7 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
   add(4 * NNof(L - 1));                              // node positions
   add(8 * np * (bw + 1));                            // band
   add(8 * np * 64);                                  // tile inverses
+  add(8 * 2 * ((bw + 63) / 64 + 1) * 64 * 64);       // factorization panels
   add(8 * (np / SB) * SB * SB, 4);                   // Dinv, DT, G, H
   add(8 * np, 2);                                    // solve vectors
   add(8LL * std::max(kMaxRedBlocks * 8, kMaxPartials) + 4096 * 4);  // reductions, scalars, flags
@@ -410,6 +411,10 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   build_levels(P, vox);
   Level& F = P.lv[0];
   const long long NN = F.NN;
+  if (3 * NN >= (1LL << 31)) {  // the fine-level kernels index DOFs with 32-bit offsets
+    P.err = "model too large: 3 x (nely+1)(nelx+1)(nelz+1) must stay below 2^31";
+    return T3X_ERR_INPUT;
+  }
   // ---- boundary conditions (536-557 or explicit), 1294-1336
   std::vector<long long> fixD, loadD;
   std::vector<uint8_t> fs;
@@ -587,6 +592,7 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     P.cs.plan(P.cN, P.cBW);
     P.band = P.alloc<double>(P.cs.band_len());
     P.tinv = P.alloc<double>(P.cs.tinv_len());
+    P.cPan = P.alloc<double>(P.cs.panel_len());
     P.cDinv = P.alloc<double>(P.cs.blk_len());
     P.cDT = P.alloc<double>(P.cs.blk_len());
     P.cG = P.alloc<double>(P.cs.blk_len());
@@ -647,7 +653,7 @@ int assemble(t3x_problem& P) {
   launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, P.band, (long long)P.cs.band_len());
   P.cs.pad_identity(s, P.band);
   T3X_HIP(hipMemsetAsync(P.dStatus, 0, sizeof(int), s));
-  P.cs.factor(s, P.band, P.tinv, P.cDinv, P.cDT, P.cG, P.cH, P.dStatus);
+  P.cs.factor(s, P.band, P.tinv, P.cPan, P.cDinv, P.cDT, P.cG, P.cH, P.dStatus);
   P.tend(TG_ASSEMBLE, ea);
   int st = 0;
   T3X_HIP(hipMemcpyAsync(&st, P.dStatus, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -279,18 +279,25 @@ void launch_diag_coarse(hipStream_t s, const Geo& g, const double* Ks, const uin
 // Level 2 (span 4, no BC child): Ks_c = sum_ss E_ss T_ss, T_ss = Q_ss^T Ke Q_ss
 // precomputed on the host (64 x 300 upper-triangle entries, 950-954, 972-976).
 // =========================================================================
-constexpr int kL2Cells = 16;  // coarse cells per block
+// As a GEMM on v_mfma_f64_16x16x4: A = T^T (300 entries in 19 blocks of 16
+// rows; K = the span^3 children in steps of 4), B = the children's moduli of
+// 16 coarse cells (staged in LDS), D = 16 entries x 16 cells, stored straight
+// into the entry-major Ks.  A work-group owns 64 consecutive coarse cells;
+// wave w holds the A fragments of entry blocks w, w+4, ... in registers, one
+// block at a time, and applies them to the 4 groups of 16 cells.  Cells that
+// are void or padding get 0 (all their children's moduli are 0); the cells
+// with a BC child are overwritten afterwards by k_galerkin_l2_bc.
+constexpr int kL2Cells = 64;  // coarse cells per work-group
+typedef double l2d4 __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(kBlock) void k_galerkin_l2(Geo gf, Geo gc, int span, const double* __restrict__ T,
-                                                        const double* __restrict__ E,
-                                                        const uint8_t* __restrict__ emaskC,
-                                                        const uint8_t* __restrict__ bcFlagC, double* __restrict__ Ks) {
-  const int s3 = span * span * span;  // 64
-  __shared__ double shE[kL2Cells][64];
+__global__ __launch_bounds__(256) void k_galerkin_l2(Geo gf, Geo gc, int span, const double* __restrict__ T,
+                                                     const double* __restrict__ E, double* __restrict__ Ks) {
+  const int s3 = span * span * span;  // 64 (span 4) or 8 (span 2)
+  __shared__ double sE[64][kL2Cells + 1];  // [child][cell]
   const long long c0 = (long long)blockIdx.x * kL2Cells;
-  for (int t = threadIdx.x; t < kL2Cells * s3; t += kBlock) {
-    const int ci = t / s3, ss = t % s3;
-    const long long cc = c0 + ci;
+  for (int t = threadIdx.x; t < s3 * kL2Cells; t += 256) {
+    const int j = t % kL2Cells, ss = t / kL2Cells;
+    const long long cc = c0 + j;
     double v = 0.0;
     if (cc < gc.NN) {
       int cy, cx, cz;
@@ -300,32 +307,42 @@ __global__ __launch_bounds__(kBlock) void k_galerkin_l2(Geo gf, Geo gc, int span
         v = E[nidx(gf, span * cy + ly, span * cx + lx, span * cz + lz)];
       }
     }
-    shE[ci][ss] = v;
+    sE[ss][j] = v;
   }
   __syncthreads();
-  const int ci = threadIdx.x / 16, chunk = threadIdx.x % 16;
-  const long long cc = c0 + ci;
-  if (cc >= gc.NN || !emaskC[cc] || bcFlagC[cc]) return;
-  const int k0 = chunk * 19, k1 = min(kTri, k0 + 19);
-  double acc[19];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane & 15, kq = lane >> 4;
+  const int nkb = s3 / 4;
+  for (int m = wave; m * 16 < kTri; m += 4) {
+    const int row = 16 * m + i;
+    double a[16];
 #pragma unroll
-  for (int i = 0; i < 19; ++i) acc[i] = 0.0;
-  for (int ss = 0; ss < s3; ++ss) {
-    const double e = shE[ci][ss];
-    const double* Tr = T + (size_t)ss * kTri + k0;
+    for (int kb = 0; kb < 16; ++kb) a[kb] = (kb < nkb && row < kTri) ? T[(size_t)(4 * kb + kq) * kTri + row] : 0.0;
 #pragma unroll
-    for (int i = 0; i < 19; ++i)
-      if (k0 + i < k1) acc[i] += Tr[i] * e;
+    for (int grp = 0; grp < kL2Cells / 16; ++grp) {
+      l2d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < 16; ++kb)
+        if (kb < nkb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kb], sE[4 * kb + kq][16 * grp + i], acc, 0, 0, 0);
+      // D[r][c] at lane c + 16 (r % 4), register r / 4
+      const long long cc = c0 + 16 * grp + i;
+      if (cc < gc.NN) {
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+          const int k = 16 * m + kq + 4 * r4;
+          if (k < kTri) Ks[(size_t)k * gc.NN + cc] = acc[r4];
+        }
+      }
+    }
   }
-#pragma unroll
-  for (int i = 0; i < 19; ++i)
-    if (k0 + i < k1) Ks[(size_t)(k0 + i) * gc.NN + cc] = acc[i];
 }
 
 void launch_galerkin_l2(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* T, const double* E,
                         const uint8_t* emaskC, const uint8_t* bcFlagC, double* Ks) {
+  (void)emaskC;
+  (void)bcFlagC;
   const int nb = (int)((gc.NN + kL2Cells - 1) / kL2Cells);
-  hipLaunchKernelGGL(k_galerkin_l2, dim3(nb), dim3(kBlock), 0, s, gf, gc, span, T, E, emaskC, bcFlagC, Ks);
+  hipLaunchKernelGGL(k_galerkin_l2, dim3(nb), dim3(256), 0, s, gf, gc, span, T, E, Ks);
 }
 
 // Interpolation weight of coarse local node J at fine child-local node a for
@@ -421,34 +438,81 @@ void launch_galerkin_l2_bc(hipStream_t s, const Geo& gf, const Geo& gc, int span
 }
 
 // Level >= 3 (span 2): Ks_c = sum over 8 children of Q^T Ks_child Q (980-995).
-__global__ __launch_bounds__(64) void k_galerkin_coarse(Geo gf, Geo gc, const double* __restrict__ KsF,
-                                                        const uint8_t* __restrict__ emaskF,
-                                                        const uint8_t* __restrict__ emaskC, double* __restrict__ KsC) {
-  __shared__ double shK[24][24];
-  __shared__ double shT[24][24];
-  const long long cc = blockIdx.x;
-  if (cc >= gc.NN || !emaskC[cc]) return;
-  int cy, cx, cz;
-  node_yxz(cc, gc, cy, cx, cz);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+// With Q = N (x) I3 (N: 8x8 hat weights of the child position) every 8x8
+// component block (c, c') of the result is N^T K^{cc'} N; the 6 blocks with
+// c <= c' give the 300 stored entries.  A work-group takes 16 consecutive
+// coarse cells, one thread per (cell, block, column Q), the 48 threads of a
+// cell adjacent (so a wave's LDS reads of a child matrix are broadcasts): per
+// child it forms the column T = K^{cc'} N(:, Q) in registers and adds N^T T
+// to its 8 accumulators.  The children's matrices are staged in LDS
+// (expanded to 24x24, padded), with global loads shared by neighbouring cells.
+constexpr int kGcCells = 16;
+constexpr int kGcThreads = kGcCells * 48;
+__global__ __launch_bounds__(kGcThreads) void k_galerkin_coarse(Geo gf, Geo gc, const double* __restrict__ KsF,
+                                                                double* __restrict__ KsC) {
+  __shared__ double sK[kGcCells][24][25];
+  __shared__ double sN[8][8];
+  const long long c0 = (long long)blockIdx.x * kGcCells;
+  const int tid = threadIdx.x, j = tid / 48, ci = tid % 48;
+  const int pr = ci / 8, Q = ci % 8;  // block pr: (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+  const int c = pr < 3 ? pr : (pr == 5 ? 1 : 0);
+  const int c2 = pr < 3 ? pr : (pr == 3 ? 1 : 2);
+  const long long cc = c0 + j;
+  double acc[8];
+#pragma unroll
+  for (int P = 0; P < 8; ++P) acc[P] = 0.0;
   for (int ch = 0; ch < 8; ++ch) {
     const int ly = ch & 1, lx = (ch >> 1) & 1, lz = ch >> 2;
-    const long long fe = nidx(gf, 2 * cy + ly, 2 * cx + lx, 2 * cz + lz);
-    if (!emaskF[fe]) continue;
-    for (int t = threadIdx.x; t < 576; t += 64) {
-      const int p = t / 24, q = t % 24;
-      shK[p][q] = KsF[(size_t)tri_index(p, q) * gf.NN + fe];
+    __syncthreads();  // previous child's sK and sN consumed
+    for (int e2 = tid; e2 < kTri * kGcCells; e2 += kGcThreads) {
+      const int jj = e2 % kGcCells, k = e2 / kGcCells;
+      const long long ccj = c0 + jj;
+      double v = 0.0;
+      if (ccj < gc.NN) {
+        int qy, qx, qz;
+        node_yxz(ccj, gc, qy, qx, qz);
+        if (qy < gc.ny && qx < gc.nx && qz < gc.nz)
+          v = KsF[(size_t)k * gf.NN + nidx(gf, 2 * qy + ly, 2 * qx + lx, 2 * qz + lz)];
+      }
+      int q = (int)((sqrtf(8.f * k + 1.f) - 1.f) * 0.5f);  // k = tri_index(p, q), p <= q
+      q += (q + 1) * (q + 2) / 2 <= k;
+      q -= q * (q + 1) / 2 > k;
+      const int p = k - q * (q + 1) / 2;
+      sK[jj][p][q] = v;
+      sK[jj][q][p] = v;
     }
+    if (tid < 64) sN[tid / 8][tid % 8] = hat_w(2, ly, lx, lz, tid / 8, tid % 8);  // N[a][J]
     __syncthreads();
-    galerkin_child(2, ly, lx, lz, shK, shT, acc, 64);
+    double Tc[8];
+#pragma unroll
+    for (int A = 0; A < 8; ++A) {
+      double v = 0.0;
+#pragma unroll
+      for (int B = 0; B < 8; ++B) v += sK[j][3 * A + c][3 * B + c2] * sN[B][Q];
+      Tc[A] = v;
+    }
+#pragma unroll
+    for (int P = 0; P < 8; ++P) {
+      double v = acc[P];
+#pragma unroll
+      for (int A = 0; A < 8; ++A) v += sN[A][P] * Tc[A];
+      acc[P] = v;
+    }
   }
-  int i = 0;
-  for (int t = threadIdx.x; t < kTri; t += 64, ++i) KsC[(size_t)t * gc.NN + cc] = acc[i];
+  if (cc >= gc.NN) return;
+#pragma unroll
+  for (int P = 0; P < 8; ++P) {
+    if (c == c2 && P > Q) continue;  // diagonal blocks: upper triangle only
+    KsC[(size_t)tri_index(3 * P + c, 3 * Q + c2) * gc.NN + cc] = acc[P];
+  }
 }
 
 void launch_galerkin_coarse(hipStream_t s, const Geo& gf, const Geo& gc, const double* KsF, const uint8_t* emaskF,
                             const uint8_t* emaskC, double* KsC) {
-  hipLaunchKernelGGL(k_galerkin_coarse, dim3((unsigned)gc.NN), dim3(64), 0, s, gf, gc, KsF, emaskF, emaskC, KsC);
+  (void)emaskF;
+  (void)emaskC;
+  const int nb = (int)((gc.NN + kGcCells - 1) / kGcCells);
+  hipLaunchKernelGGL(k_galerkin_coarse, dim3(nb), dim3(kGcThreads), 0, s, gf, gc, KsF, KsC);
 }
 
 // Multigrid transfers (783-814) live in t3x_mg.hip.
