@@ -27,11 +27,16 @@ double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Zeroed device allocation.  The fill is ordered on `stream` and waited for:
+// a plain hipMemset runs on the null stream, which is not ordered with the
+// non-blocking problem streams, and may still be in flight (and then land
+// after the first kernel that writes the buffer) when it returns.
 template <class T>
-T* dalloc(size_t n) {
+T* dalloc(size_t n, hipStream_t stream) {
   void* p = nullptr;
   T3X_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
-  T3X_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  T3X_HIP(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), stream));
+  T3X_HIP(hipStreamSynchronize(stream));
   return (T*)p;
 }
 
@@ -209,7 +214,7 @@ struct t3x_problem {
   long long bytes = 0;  // device bytes allocated by this problem
   template <class T>
   T* alloc(size_t n) {
-    T* p = dalloc<T>(n);
+    T* p = dalloc<T>(n, stream);
     allocs.push_back(p);
     bytes += (long long)(std::max<size_t>(n, 1) * sizeof(T));
     return p;
@@ -265,6 +270,13 @@ struct t3x_problem {
   const uint8_t* design_mask() const { return active ? active : lv[0].emask; }
   bool standard() const { return P.vcycle == 1; }  // 'Standard' V-cycle (FP32)
   void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
+  // Host-to-device upload ordered on the problem stream (a plain hipMemcpy from
+  // pageable memory may return before its DMA lands and is not ordered with a
+  // non-blocking stream), waited for so the host buffer may be released.
+  void upload(void* dst, const void* src, size_t bytes) {
+    T3X_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    sync();
+  }
 };
 
 namespace {
@@ -475,8 +487,8 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     Level& V = P.lv[l];
     V.emask = P.alloc<uint8_t>(V.NN);
     V.nmask = P.alloc<uint8_t>(V.NN);
-    T3X_HIP(hipMemcpy(V.emask, V.hEmask.data(), V.NN, hipMemcpyHostToDevice));
-    T3X_HIP(hipMemcpy(V.nmask, V.hNmask.data(), V.NN, hipMemcpyHostToDevice));
+    P.upload(V.emask, V.hEmask.data(), V.NN);
+    P.upload(V.nmask, V.hNmask.data(), V.NN);
     if (l < P.L - 1) V.diag = P.alloc<double>(3 * V.NN);
     if (l >= 1) {
       V.r = P.alloc<double>(3 * V.NN);
@@ -503,28 +515,28 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     P.rtmp = P.alloc<double>(tl);
   }
   P.fix = P.alloc<uint8_t>(NN);
-  T3X_HIP(hipMemcpy(P.fix, hfix.data(), NN, hipMemcpyHostToDevice));
+  P.upload(P.fix, hfix.data(), NN);
   P.F = P.alloc<double>(3 * NN);
-  T3X_HIP(hipMemcpy(P.F, hF.data(), 3 * NN * sizeof(double), hipMemcpyHostToDevice));
+  P.upload(P.F, hF.data(), 3 * NN * sizeof(double));
   for (double** v : {&P.U, &P.r, &P.z, &P.p, &P.q, &P.tmp3}) *v = P.alloc<double>(3 * NN);
   for (double** v : {&P.E, &P.x, &P.xPhys, &P.ce, &P.dc, &P.ca, &P.cb, &P.cc, &P.cd}) *v = P.alloc<double>(NN);
   for (double** v : {&P.fb, &P.fx, &P.fr, &P.fz, &P.fp, &P.fq}) *v = P.alloc<double>(NN);
   // ---- Ke and L2 templates
   const std::vector<double> Ke = gauss_ke(P.P.nu, 1.0);
   P.dKe = P.alloc<double>(576);
-  T3X_HIP(hipMemcpy(P.dKe, Ke.data(), 576 * sizeof(double), hipMemcpyHostToDevice));
+  P.upload(P.dKe, Ke.data(), 576 * sizeof(double));
   if (P.mixed()) P.z32 = P.alloc<float>(3 * NN);
   if (P.standard()) {
     std::vector<float> Kf(Ke.begin(), Ke.end());
     P.Ke32 = P.alloc<float>(576);
-    T3X_HIP(hipMemcpy(P.Ke32, Kf.data(), 576 * sizeof(float), hipMemcpyHostToDevice));
+    P.upload(P.Ke32, Kf.data(), 576 * sizeof(float));
     P.E32 = P.alloc<float>(NN);
     P.x32 = P.alloc<float>(3 * NN);
     P.d32 = P.alloc<float>(3 * NN);
   }
   const std::vector<double> T2 = l2_templates(Ke, P.lv[1].span);
   P.dT2 = P.alloc<double>(T2.size());
-  T3X_HIP(hipMemcpy(P.dT2, T2.data(), T2.size() * sizeof(double), hipMemcpyHostToDevice));
+  P.upload(P.dT2, T2.data(), T2.size() * sizeof(double));
   // ---- L2 cells containing a fine cell that touches a fixed node (1397-1422)
   {
     const Level& C = P.lv[1];
@@ -543,9 +555,9 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       if (flag[c]) list.push_back((int)c);
     P.nBc2 = (int)list.size();
     P.dBc2 = P.alloc<int>(list.size());
-    if (!list.empty()) T3X_HIP(hipMemcpy(P.dBc2, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!list.empty()) P.upload(P.dBc2, list.data(), list.size() * sizeof(int));
     P.dBcFlag2 = P.alloc<uint8_t>(C.NN);
-    T3X_HIP(hipMemcpy(P.dBcFlag2, flag.data(), C.NN, hipMemcpyHostToDevice));
+    P.upload(P.dBcFlag2, flag.data(), C.NN);
   }
   // ---- coarsest band ordering: the axis permutation with the smallest bandwidth
   {
@@ -586,9 +598,9 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       if (bestPos[n] >= 0)
         for (int c = 0; c < 3; ++c) cmap[3 * bestPos[n] + c] = c * C.NN + n;
     P.dCMap = P.alloc<long long>(P.cN);
-    T3X_HIP(hipMemcpy(P.dCMap, cmap.data(), P.cN * sizeof(long long), hipMemcpyHostToDevice));
+    P.upload(P.dCMap, cmap.data(), P.cN * sizeof(long long));
     P.dNodePos = P.alloc<int>(C.NN);
-    T3X_HIP(hipMemcpy(P.dNodePos, bestPos.data(), C.NN * sizeof(int), hipMemcpyHostToDevice));
+    P.upload(P.dNodePos, bestPos.data(), C.NN * sizeof(int));
     P.cs.plan(P.cN, P.cBW);
     P.band = P.alloc<double>(P.cs.band_len());
     P.tinv = P.alloc<double>(P.cs.tinv_len());
@@ -610,8 +622,8 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   P.nCells = (int)P.hCellMap.size();
   P.dNodeMap = P.alloc<int>(P.nNodes);
   P.dCellMap = P.alloc<int>(P.nCells);
-  T3X_HIP(hipMemcpy(P.dNodeMap, P.hNodeMap.data(), P.nNodes * sizeof(int), hipMemcpyHostToDevice));
-  T3X_HIP(hipMemcpy(P.dCellMap, P.hCellMap.data(), P.nCells * sizeof(int), hipMemcpyHostToDevice));
+  P.upload(P.dNodeMap, P.hNodeMap.data(), P.nNodes * sizeof(int));
+  P.upload(P.dCellMap, P.hCellMap.data(), P.nCells * sizeof(int));
   // ---- reductions and scalars
   P.ws.partials = P.alloc<double>((size_t)std::max(kMaxRedBlocks * 8, kMaxPartials));
   P.ws.counter = P.alloc<unsigned int>(4);
@@ -652,13 +664,12 @@ int assemble(t3x_problem& P) {
   const Level& C = P.lv[P.L - 1];
   launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, P.band, (long long)P.cs.band_len());
   P.cs.pad_identity(s, P.band);
-  T3X_HIP(hipMemsetAsync(P.dStatus, 0, sizeof(int), s));
+  launch_set_int(s, P.dStatus, 0);
   P.cs.factor(s, P.band, P.tinv, P.cPan, P.cDinv, P.cDT, P.cG, P.cH, P.dStatus);
   P.tend(TG_ASSEMBLE, ea);
-  int st = 0;
-  T3X_HIP(hipMemcpyAsync(&st, P.dStatus, sizeof(int), hipMemcpyDeviceToHost, s));
+  T3X_HIP(hipMemcpyAsync(P.hbuf, P.dStatus, sizeof(int), hipMemcpyDeviceToHost, s));
   P.sync();
-  if (st) { P.err = "coarsest matrix is not positive definite"; return T3X_ERR_NOT_PD; }
+  if (*(const int*)P.hbuf) { P.err = "coarsest matrix is not positive definite"; return T3X_ERR_NOT_PD; }
   P.assembled = true;
   return T3X_OK;
 }
@@ -839,7 +850,7 @@ void filter_setup(t3x_problem& P, int slot, double rMin) {
   Filter& Fl = P.filt[slot];
   for (int a = 0; a < 8; ++a)
     for (int b = 0; b < 8; ++b) Fl.kernel[b * 8 + a] = ir * ir * K0[a][b] + K1[a][b];
-  T3X_HIP(hipMemcpy(P.dFiltKern + slot * 64, Fl.kernel, 64 * sizeof(double), hipMemcpyHostToDevice));
+  P.upload(P.dFiltKern + slot * 64, Fl.kernel, 64 * sizeof(double));
   if (!Fl.diagInv) Fl.diagInv = P.alloc<double>(P.NN());
   launch_filter_diag(P.stream, geo(P.lv[0]), P.dFiltKern + slot * 64, P.lv[0].emask, P.lv[0].nmask, Fl.diagInv);
   Fl.ready = true;
@@ -898,9 +909,8 @@ double masked_sum(t3x_problem& P, const double* a, const double* b, int mode) {
 double oc_update(t3x_problem& P, double* x, const double* xPhys, const double* dc, double V0, int* nBis) {
   hipStream_t s = P.stream;
   const double fval = masked_sum(P, xPhys, nullptr, 0) / (double)P.nCells - V0;  // 145
-  double init[8] = {0.0, 1e9, fval, 0.0, 0.0, 0.0, 0.0, 0.0};                  // 146
-  std::memcpy(P.hbuf, init, sizeof(init));
-  T3X_HIP(hipMemcpyAsync(P.oc, P.hbuf, sizeof(init), hipMemcpyHostToDevice, s));
+  const double init[8] = {0.0, 1e9, fval, 0.0, 0.0, 0.0, 0.0, 0.0};            // 146
+  launch_set_vals(s, P.oc, 8, init);
   for (int pass = 0; pass < 64; ++pass) {
     launch_oc_pass(s, P.NN(), P.lv[0].emask, P.passive, x, dc, P.ws, P.oc, 3);
     if (pass % 8 == 7) {
@@ -929,8 +939,7 @@ void mma_alloc(t3x_problem& P) {
 int mma_update(t3x_problem& P, int iter, const double* dfdx, double gx, const double* dgdx) {
   hipStream_t s = P.stream;
   const long long NN = P.NN();
-  P.hbuf[0] = gx;
-  T3X_HIP(hipMemcpyAsync(P.mm + MM_GX, P.hbuf, sizeof(double), hipMemcpyHostToDevice, s));
+  launch_set_vals(s, P.mm + MM_GX, 1, &gx);
   const uint8_t* am = P.design_mask();
   launch_mma_gensub(s, NN, am, iter, P.mx, P.mxmin, P.mxmax, P.mxold1, P.mxold2, P.mlow, P.mupp, dfdx, dgdx,
                     P.malpha, P.mbeta, P.mp0, P.mq0, P.mpij, P.mqij, P.ws, P.mm, (double)(P.nCells - P.nPassive));
@@ -1309,8 +1318,8 @@ int t3x_stream_copy(long long bytes, int reps, double* msPerCopy) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = T3X_OK;
   try {
-    a = dalloc<double>(n);
-    b = dalloc<double>(n);
+    a = dalloc<double>(n, s);
+    b = dalloc<double>(n, s);
     T3X_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     T3X_HIP(hipEventCreate(&e0));
     T3X_HIP(hipEventCreate(&e1));
@@ -1353,8 +1362,8 @@ int t3x_set_passive(t3x_problem* h, int n, const int* voxelIds) {
       h->passive = h->alloc<uint8_t>(F.NN);
       h->active = h->alloc<uint8_t>(F.NN);
     }
-    T3X_HIP(hipMemcpy(h->passive, pas.data(), F.NN, hipMemcpyHostToDevice));
-    T3X_HIP(hipMemcpy(h->active, act.data(), F.NN, hipMemcpyHostToDevice));
+    h->upload(h->passive, pas.data(), F.NN);
+    h->upload(h->active, act.data(), F.NN);
     h->nPassive = cnt;
     return (int)T3X_OK;
   });

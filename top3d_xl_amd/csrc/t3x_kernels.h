@@ -110,6 +110,10 @@ void launch_band_solve(hipStream_t s, int n, int bw, int nb, const double* band,
 
 // ---------------------------------------------------------------- vectors
 void launch_fill(hipStream_t s, double* p, double v, long long n);
+// dst[0..n) = vals[0..n), n <= 8, as kernel arguments: host scalars reach the
+// device in stream order without a host-to-device copy from host memory.
+void launch_set_vals(hipStream_t s, double* dst, int n, const double* vals);
+void launch_set_int(hipStream_t s, int* dst, int v);
 void launch_copy(hipStream_t s, double* dst, const double* src, long long n);
 void launch_sub(hipStream_t s, double* r, const double* b, const double* q, long long n);  // r = b - q
 void launch_norm2(hipStream_t s, const double* v, long long n, const RedWs& ws, double* out);

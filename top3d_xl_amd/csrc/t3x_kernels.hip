@@ -726,6 +726,19 @@ void launch_band_solve(hipStream_t s, int n, int bw, int nb, const double* band,
 __global__ void k_fill(double* p, double v, long long n) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) p[i] = v;
 }
+struct Vals8 {
+  double v[8];
+};
+__global__ void k_set_vals(double* dst, int n, Vals8 vals) {
+  if ((int)threadIdx.x < n) dst[threadIdx.x] = vals.v[threadIdx.x];
+}
+void launch_set_vals(hipStream_t s, double* dst, int n, const double* vals) {
+  Vals8 v{};
+  for (int i = 0; i < n && i < 8; ++i) v.v[i] = vals[i];
+  hipLaunchKernelGGL(k_set_vals, dim3(1), dim3(64), 0, s, dst, n < 8 ? n : 8, v);
+}
+__global__ void k_set_int(int* dst, int v) { *dst = v; }
+void launch_set_int(hipStream_t s, int* dst, int v) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dst, v); }
 void launch_fill(hipStream_t s, double* p, double v, long long n) {
   hipLaunchKernelGGL(k_fill, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, p, v, n);
 }
