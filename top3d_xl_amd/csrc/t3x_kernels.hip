@@ -940,34 +940,96 @@ void launch_filter_e2n(hipStream_t s, const Geo& g, const uint8_t* emask, const 
   hipLaunchKernelGGL(k_filter_e2n, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, emask, src, b);
 }
 
-__global__ __launch_bounds__(kBlock) void k_filter_apply(Geo g, const double* __restrict__ kern,
-                                                         const uint8_t* __restrict__ emask,
-                                                         const double* __restrict__ u, double* __restrict__ y,
-                                                         RedWs ws, double* sc, const int* fl, int pcgStep) {
+// y = K_F u for the PDE filter's nodal operator: per node, the rows of the
+// 8x8 filter kernel from its active cells.  3-D grid of 64x4 node columns,
+// each thread walking zc node layers with a sliding 3x3x3 window of u and the
+// masks of the 2x2x2 cells around the node in registers (32-bit offsets:
+// 3 NN < 2^31 is checked at set-up).  pcgStep: per-block u.y partials.
+constexpr int kFaY = 64, kFaX = 4;
+__global__ __launch_bounds__(kFaY * kFaX) void k_filter_apply(Geo g, int zc, const double* __restrict__ kern,
+                                                              const uint8_t* __restrict__ emask,
+                                                              const double* __restrict__ u, double* __restrict__ y,
+                                                              RedWs ws, const int* fl, int pcgStep) {
   if (fl && fl[FL_DONE]) return;
+  const int iy = blockIdx.x * kFaY + threadIdx.x, ix = blockIdx.y * kFaX + threadIdx.y;
+  const int z0 = blockIdx.z * zc, z1 = min(g.NZ, z0 + zc);
+  const bool own = iy < g.NY && ix < g.NX;
+  const int NY = g.NY, plane = g.NY * g.NX;
+  // window: w[l][ky + 3 kx] = u(iy-1+ky, ix-1+kx, z-1+l); m[l][cy + 2 cx] = active cell (iy-1+cy, ix-1+cx, z-1+l)
+  double w[3][9];
+  bool m[2][4];
+  auto load_u = [&](int z, double (&wl)[9]) {
+#pragma unroll
+    for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky) {
+        const int yy = iy - 1 + ky, xx = ix - 1 + kx;
+        const bool in = own && z >= 0 && z < g.NZ && yy >= 0 && xx >= 0 && yy < g.NY && xx < g.NX;
+        wl[ky + 3 * kx] = in ? u[yy + NY * xx + plane * z] : 0.0;
+      }
+  };
+  auto load_m = [&](int z, bool (&ml)[4]) {
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy) {
+        const int yy = iy - 1 + cy, xx = ix - 1 + cx;
+        const bool in = own && z >= 0 && z < g.nz && yy >= 0 && xx >= 0 && yy < g.ny && xx < g.nx;
+        ml[cy + 2 * cx] = in && emask[yy + NY * xx + plane * z];
+      }
+  };
+  load_u(z0 - 1, w[0]);
+  load_u(z0, w[1]);
+  load_m(z0 - 1, m[0]);
   double pq = 0.0;
-  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < g.NN; n += (long long)gridDim.x * kBlock) {
-    int iy, ix, iz;
-    node_yxz(n, g, iy, ix, iz);
-    double acc = 0.0;
+  for (int z = z0; z < z1; ++z) {
+    load_u(z + 1, w[2]);
+    load_m(z, m[1]);
+    if (own) {
+      double acc = 0.0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
-      const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
-      if (!cell_in(g, cy, cx, cz)) continue;
-      if (!emask[nidx(g, cy, cx, cz)]) continue;
-      const int jn = loc_index(dy, dx, dz);
-      double s = 0.0;
+      for (int c = 0; c < 8; ++c) {  // cell (iy-dy, ix-dx, z-dz)
+        const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;
+        if (!m[1 - dz][(1 - dy) + 2 * (1 - dx)]) continue;
+        const int jn = loc_index(dy, dx, dz);
+        double sc = 0.0;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) s += u[nidx(g, cy + kLocDy[b], cx + kLocDx[b], cz + kLocDz[b])] * kern[jn * 8 + b];
-      acc += s;
+        for (int bq = 0; bq < 8; ++bq) {
+          const int ky = 1 - dy + kLocDy[bq], kx = 1 - dx + kLocDx[bq], l = 1 - dz + kLocDz[bq];
+          sc += w[l][ky + 3 * kx] * kern[jn * 8 + bq];
+        }
+        acc += sc;
+      }
+      y[iy + NY * ix + plane * z] = acc;
+      pq += w[1][4] * acc;
     }
-    y[n] = acc;
-    pq += u[n] * acc;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      w[0][i] = w[1][i];
+      w[1][i] = w[2][i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m[0][i] = m[1][i];
   }
   if (pcgStep) {
-    double v[1] = {pq};
-    write_partials<1>(v, ws.partials);
+    __shared__ double sh[kFaY * kFaX / 64];
+    const int t = threadIdx.x + kFaY * threadIdx.y;
+    const double v = wave_sum(pq);
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+      double s2 = 0.0;
+      for (int k = 0; k < kFaY * kFaX / 64; ++k) s2 += sh[k];
+      ws.partials[blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z)] = s2;
+    }
+  }
+}
+static dim3 filter_grid(const Geo& g, int& zc) {
+  zc = 8;
+  for (;;) {
+    const dim3 d((g.NY + kFaY - 1) / kFaY, (g.NX + kFaX - 1) / kFaX, (g.NZ + zc - 1) / zc);
+    if ((long long)d.x * d.y * d.z <= kMaxPartials) return d;
+    zc *= 2;
   }
 }
 void launch_filter_apply(hipStream_t s, const Geo& g, const double* kern, const uint8_t* emask, const double* u,
@@ -975,9 +1037,10 @@ void launch_filter_apply(hipStream_t s, const Geo& g, const double* kern, const 
   RedWs w{nullptr, nullptr};
   int step = 0;
   if (ws) { w = *ws; step = 1; }
-  const int nb = step ? red_blocks(g.NN) : grid_for(g.NN);
-  hipLaunchKernelGGL(k_filter_apply, dim3(nb), dim3(kBlock), 0, s, g, kern, emask, u, y, w, sc, fl, step);
-  if (step) fin1(s, FIN_PQ, nb, w, sc, (int*)fl, nullptr);
+  int zc = 8;
+  const dim3 grid = filter_grid(g, zc);
+  hipLaunchKernelGGL(k_filter_apply, grid, dim3(kFaY, kFaX), 0, s, g, zc, kern, emask, u, y, w, fl, step);
+  if (step) fin1(s, FIN_PQ, (int)(grid.x * grid.y * grid.z), w, sc, (int*)fl, nullptr);
 }
 
 __global__ void k_filter_n2e(Geo g, const uint8_t* __restrict__ emask, const double* __restrict__ u,
@@ -1068,12 +1131,13 @@ void launch_masked_sum(hipStream_t s, long long n, const uint8_t* emask, const d
 // midpoints of the next `depth` bisection levels; the last block replays the
 // sequential decisions with identical arithmetic.
 // oc[]: 0 l1, 1 l2, 2 fval, 3 done, 4 lastLmid, 5 nBisect, 6 change
-constexpr int kOcDepth = 3;
+constexpr int kOcDepth = 4;
 constexpr int kOcCand = (1 << kOcDepth) - 1;
 
-__device__ __forceinline__ double oc_xnew(double x, double dc, double lmid) {
+// x .* sqrt(-dc ./ lmid) (149) as t * rl with t = x sqrt(-dc) formed once per
+// cell and rl = 1/sqrt(lmid) once per candidate, then the move limits (150).
+__device__ __forceinline__ double oc_clamp(double x, double v) {
   const double move = 0.2;
-  double v = x * sqrt(-dc / 1.0 / lmid);  // 149
   v = fmin(x + move, v);
   v = fmin(1.0, v);
   v = fmax(x - move, v);
@@ -1099,16 +1163,20 @@ __global__ __launch_bounds__(kBlock) void k_oc_pass(long long n, const uint8_t* 
       Bv[2 * k + 1] = Bv[k];
     }
   }
-  double s[kOcCand];
+  double s[kOcCand], rl[kOcCand];
 #pragma unroll
-  for (int k = 0; k < kOcCand; ++k) s[k] = 0.0;
+  for (int k = 0; k < kOcCand; ++k) {
+    s[k] = 0.0;
+    rl[k] = 1.0 / sqrt(lm[k + 1]);
+  }
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
     if (!emask[i]) continue;
-    const double xi = x[i], di = dc[i];
+    const double xi = x[i];
+    const double t = xi * sqrt(-dc[i]);
     const bool pas = passive && passive[i];
 #pragma unroll
     for (int k = 0; k < kOcCand; ++k) {
-      const double xn = pas ? 1.0 : oc_xnew(xi, di, lm[k + 1]);
+      const double xn = pas ? 1.0 : oc_clamp(xi, t * rl[k]);
       s[k] += 1.0 * (xn - xi);  // 151
     }
   }
@@ -1152,12 +1220,12 @@ void launch_oc_pass(hipStream_t s, long long n, const uint8_t* emask, const uint
 __global__ __launch_bounds__(kBlock) void k_oc_final(long long n, const uint8_t* __restrict__ emask,
                                                      const uint8_t* __restrict__ passive, double* __restrict__ x,
                                                      const double* __restrict__ dc, RedWs ws, double* oc) {
-  const double lmid = oc[4];
+  const double rl = 1.0 / sqrt(oc[4]);
   double m = 0.0;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
     if (!emask[i]) continue;
     const double xi = x[i];
-    const double xn = (passive && passive[i]) ? 1.0 : oc_xnew(xi, dc[i], lmid);
+    const double xn = (passive && passive[i]) ? 1.0 : oc_clamp(xi, xi * sqrt(-dc[i]) * rl);
     m = fmax(m, fabs(xn - xi));
     x[i] = xn;
   }
