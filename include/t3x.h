@@ -53,8 +53,10 @@ typedef struct t3x_params {
   int checkEvery;        /* PCG convergence flag read-back interval (>=1) */
   int timing;            /* 1 = time kernel groups with HIP events (t3x_kernel_time) */
   int vcycle;            /* typeVcycle_ (34): 0 = 'Adapted' (747-781); 1 = 'Standard'
-                            (V(1,1) damped Jacobi with K*u on every level, 759-777),
-                            FP32, available with precision = 1 */
+                            (V(1,1) damped Jacobi with K*u on every level, 759-777;
+                            coarse levels through the stored Galerkin matrices,
+                            727-744): FP64 with precision = 0, FP32 level
+                            vectors and matrices with precision = 1 */
 } t3x_params;
 
 /* Fills the reference defaults (InitialSettings, 21-38). */
@@ -75,6 +77,23 @@ void t3x_default_params(t3x_params* p);
 int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
                const uint8_t* fixState, int nLoad, const int* loadNodes, const double* loadVec,
                const t3x_params* params, t3x_problem** out);
+/*
+ * t3x_create with several load cases (loadingCond_{1..n}, the .TopVoxel
+ * "Additional Loads" of 483-493; F_ columns, 1315-1321).  The cases are
+ * concatenated in loadNodes/loadVec, case ii holding nPerCase[ii] nodes.
+ * A node loaded in any case is released from the fixings (1301-1305).
+ * Each design iteration solves one warm-started PCG per case (101, 121,
+ * 301, 321) and the element compliance is sum_j ce_j / nCases
+ * (objWeightingList_, 493, 597).  nFix < 0 (the built-in cuboid) ignores
+ * the load arguments and has one case of weight 1 (557-558).
+ * t3x_result.lssIts and .resHist then hold nLoop*nCases entries, one per
+ * solve, case fastest (lssIts(end+1), 121); t3x_iter.pcgIts is the sum over
+ * the cases and t3x_iter.res the largest final residual.
+ */
+int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                     const uint8_t* fixState, int nCases, const int* nPerCase, const int* loadNodes,
+                     const double* loadVec, const t3x_params* params, t3x_problem** out);
+int t3x_num_load_cases(const t3x_problem* h);
 void t3x_destroy(t3x_problem* h);
 const char* t3x_last_error(const t3x_problem* h);
 /* Passive elements (passiveElements_, 472, 524-527): voxel ids (0-based,
@@ -95,11 +114,11 @@ typedef struct t3x_result {
   double* volHist;
   double* consHist;
   double* sharpHist;
-  int* lssIts;         /* PCG iterations per iteration (121) */
+  int* lssIts;         /* PCG iterations of each solve (121): nLoop x nCases, case fastest */
   double* tHist;
   double* xSnap;       /* nSnap x numElements: xPhys after each of the first nSnap iterations */
   int nSnap;
-  double* resHist;     /* final relative PCG residual of each elasticity solve (618) */
+  double* resHist;     /* final relative PCG residual of each elasticity solve (618), as lssIts */
   double cSolid;       /* out: compliance of the fully solid domain (104) */
   int loops;           /* out: iterations run */
   int warnings;        /* out: PCG max-iteration warnings */

@@ -18,7 +18,7 @@ function TOP3D_XL(inputModel, consType, V0, nLoop, rMin, varargin)
 	params = struct('precision', 0); %% 0: FP64 plain path (1: mixed precision)
 	if ischar(inputModel) || isstring(inputModel)
 		M = ReadTopVoxel(inputModel);
-		h = t3x_mex('create', M.voxels, M.fix(:,1), M.fix(:,2:4), M.load(:,1), M.load(:,2:4), params);
+		h = t3x_mex('create', M.voxels, M.fix(:,1), M.fix(:,2:4), M.load(:,1), M.load(:,2:4), params, M.extraLoads);
 		if ~isempty(M.passive), t3x_mex('passive', h, M.passive); end
 	elseif islogical(inputModel)
 		h = t3x_mex('create', inputModel, params);
@@ -64,6 +64,17 @@ function M = ReadTopVoxel(fileName)
 	M.fix = zeros(0, 4); if nFix > 0, M.fix = fscanf(fid, '%d %d %d %d', [4 nFix])'; end
 	fscanf(fid, '%s', 1); nLoad = fscanf(fid, '%d', 1);
 	M.load = zeros(0, 4); if nLoad > 0, M.load = fscanf(fid, '%d %e %e %e', [4 nLoad])'; end
+	%% Additional Loads (483-492): case idx goes to loadingCond_{idx}
+	M.extraLoads = {};
+	if ~isempty(fscanf(fid, '%s %s', 2))
+		nAdd = fscanf(fid, '%d', 1);
+		for ii = 1:nAdd
+			if ~strcmp(fscanf(fid, '%s', 1), 'Loads:'), fclose(fid); error('Incompatible Mesh Data Format!'); end
+			n = fscanf(fid, '%d', 1); idx = fscanf(fid, '%d', 1);
+			iLoad = fscanf(fid, '%d %e %e %e', [4 n])';
+			[~, ia] = unique(iLoad(:,1)); M.extraLoads{idx-1} = iLoad(ia,:);
+		end
+	end
 	fclose(fid);
 	[~, ia] = unique(M.fix(:,1)); M.fix = M.fix(ia,:); M.fix(:,2:4) = double(M.fix(:,2:4) == 1);
 	[~, ia] = unique(M.load(:,1)); M.load = M.load(ia,:);

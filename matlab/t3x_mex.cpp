@@ -10,6 +10,9 @@
 //                                                      explicit BCs, 1-based node ids of the
 //                                                      unpadded (nely+1)x(nelx+1)x(nelz+1) grid
 //   h = t3x_mex(..., params)                           optional struct with t3x_params fields
+//   h = t3x_mex('create', voxels, fixNodes, fixState, loadNodes, loadVec, params, extraLoads)
+//                                                      further load cases (loadingCond_{2..}, 483-493):
+//                                                      a cell array of n x 4 [node Fx Fy Fz] matrices
 //   t3x_mex('passive', h, voxelIds)                   passive elements, 1-based voxel ids
 //   out = t3x_mex('run', h, consType, V0, nLoop, rMin [, rHat])
 //   t3x_mex('destroy', h)
@@ -95,7 +98,7 @@ void cmd_create(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   }
   t3x_params P;
   const bool explicitBC = nrhs >= 6;
-  read_params(nrhs == 3 ? prhs[2] : (nrhs == 7 ? prhs[6] : nullptr), &P);
+  read_params(nrhs == 3 ? prhs[2] : (nrhs >= 7 ? prhs[6] : nullptr), &P);
   t3x_problem* h = nullptr;
   int rc;
   if (explicitBC) {
@@ -116,8 +119,22 @@ void cmd_create(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
       ln[i] = (int)l[i] - 1;
       for (int c = 0; c < 3; ++c) lv[3 * i + c] = lvd[i + nl * c];       // nl x 3
     }
-    rc = t3x_create(v.data(), nely, nelx, nelz, (int)nf, fn.data(), fs.data(), (int)nl, ln.data(), lv.data(), &P,
-                    &h);
+    std::vector<int> npc(1, (int)nl);
+    if (nrhs >= 8 && mxIsCell(prhs[7])) {
+      for (size_t k = 0; k < mxGetNumberOfElements(prhs[7]); ++k) {
+        const mxArray* c = mxGetCell(prhs[7], k);
+        if (!c || mxGetN(c) != 4) mexErrMsgIdAndTxt("t3x:input", "Incompatible Mesh Data Format!");
+        const size_t m = mxGetM(c);
+        const double* a = mxGetDoubles(c);  // m x 4, column-major
+        for (size_t i = 0; i < m; ++i) {
+          ln.push_back((int)a[i] - 1);
+          for (int cc = 0; cc < 3; ++cc) lv.push_back(a[i + m * (cc + 1)]);
+        }
+        npc.push_back((int)m);
+      }
+    }
+    rc = t3x_create_multi(v.data(), nely, nelx, nelz, (int)nf, fn.data(), fs.data(), (int)npc.size(), npc.data(),
+                          ln.data(), lv.data(), &P, &h);
   } else {
     rc = t3x_create(v.data(), nely, nelx, nelz, -1, nullptr, nullptr, 0, nullptr, nullptr, &P, &h);
   }
@@ -155,8 +172,10 @@ void cmd_run(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   int info[1 + 8 * 16];
   t3x_info(h, info);
   const int nE = info[1 + 4];
-  std::vector<double> xPhys(nE), cH(nLoop), vH(nLoop), conH(nLoop), sH(nLoop), tH(6 * (size_t)nLoop), rH(nLoop);
-  std::vector<int> its(nLoop);
+  const int nc = t3x_num_load_cases(h);
+  std::vector<double> xPhys(nE), cH(nLoop), vH(nLoop), conH(nLoop), sH(nLoop), tH(6 * (size_t)nLoop),
+      rH((size_t)nLoop * nc);
+  std::vector<int> its((size_t)nLoop * nc);  // one entry per solve (lssIts(end+1), 121)
   t3x_result R{};
   R.xPhys = xPhys.data(); R.cHist = cH.data(); R.volHist = vH.data(); R.consHist = conH.data();
   R.sharpHist = sH.data(); R.lssIts = its.data(); R.tHist = tH.data(); R.resHist = rH.data();
@@ -171,8 +190,8 @@ void cmd_run(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   mxSetField(out, 0, "volHist", dvec(vH.data(), k));
   mxSetField(out, 0, "consHist", dvec(conH.data(), k));
   mxSetField(out, 0, "sharpHist", dvec(sH.data(), k));
-  std::vector<double> itsd(its.begin(), its.begin() + k);
-  mxSetField(out, 0, "lssIts", dvec(itsd.data(), k));
+  std::vector<double> itsd(its.begin(), its.begin() + (size_t)k * nc);
+  mxSetField(out, 0, "lssIts", dvec(itsd.data(), k * nc));
   mxArray* T = mxCreateDoubleMatrix(k, 6, mxREAL);  // row-major nLoop x 6 -> MATLAB column-major
   for (int i = 0; i < k; ++i)
     for (int c = 0; c < 6; ++c) mxGetDoubles(T)[i + (size_t)k * c] = tH[6 * (size_t)i + c];

@@ -34,6 +34,9 @@ def _load():
     L.orc_create.restype = vp
     L.orc_create.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                              C.c_int, C.c_void_p, C.c_void_p]
+    L.orc_create_multi.restype = vp
+    L.orc_create_multi.argtypes = [_u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     L.orc_destroy.argtypes = [vp]
     L.orc_last_error.restype = C.c_char_p
     L.orc_last_error.argtypes = [vp]
@@ -63,6 +66,9 @@ def _load():
     L.orc_num_threads.restype = C.c_int
     L.orc_set_order_variant.argtypes = [vp, C.c_int]
     L.orc_set_passive.argtypes = [vp, C.c_int, _ip]
+    L.orc_set_vcycle.argtypes = [vp, C.c_int]
+    L.orc_set_omega.argtypes = [vp, C.c_double]
+    L.orc_apply_K_level.argtypes = [vp, C.c_int, _dp, _dp]
     _lib = L
     return L
 
@@ -75,7 +81,9 @@ def cuboid_bc_oracle(nely: int, nelx: int, nelz: int):
 class Oracle:
     """One oracle model (TOP3D_XL.m state after CreateVoxelFEAmodel + setup)."""
 
-    def __init__(self, voxels: np.ndarray, fix=None, load=None):
+    def __init__(self, voxels: np.ndarray, fix=None, load=None, extra_loads=()):
+        """extra_loads: further load cases (nodes, vec) after `load`
+        (loadingCond_{2..}, 483-493)."""
         L = _load()
         vox = np.asfortranarray(voxels.astype(np.uint8))
         nely, nelx, nelz = vox.shape
@@ -85,14 +93,16 @@ class Oracle:
             h = L.orc_create(flat, nely, nelx, nelz, -1, None, None, 0, None, None)
         else:
             fn, fs = fix
-            ln, lv = load
+            cases = [load] + list(extra_loads)
             fn = np.ascontiguousarray(fn, dtype=np.int32)
             fs = np.ascontiguousarray(fs, dtype=np.uint8).reshape(-1)
-            ln = np.ascontiguousarray(ln, dtype=np.int32)
-            lv = np.ascontiguousarray(lv, dtype=np.float64).reshape(-1)
-            self._keep += [fn, fs, ln, lv]
-            h = L.orc_create(flat, nely, nelx, nelz, len(fn), fn.ctypes.data, fs.ctypes.data,
-                             len(ln), ln.ctypes.data, lv.ctypes.data)
+            npc = np.array([len(c[0]) for c in cases], dtype=np.int32)
+            ln = np.ascontiguousarray(np.concatenate([np.asarray(c[0]).reshape(-1) for c in cases]), dtype=np.int32)
+            lv = np.ascontiguousarray(np.concatenate([np.asarray(c[1], dtype=np.float64).reshape(-1)
+                                                      for c in cases]), dtype=np.float64)
+            self._keep += [fn, fs, npc, ln, lv]
+            h = L.orc_create_multi(flat, nely, nelx, nelz, len(fn), fn.ctypes.data, fs.ctypes.data,
+                                   len(cases), npc.ctypes.data, ln.ctypes.data, lv.ctypes.data)
         self.h = h
         self.L = L
         if not L.orc_ok(h):
@@ -111,6 +121,7 @@ class Oracle:
                                     intermediateNumNodes=int(o[6])))
         self.nE = self.levels[0]["numElements"]
         self.nN = self.levels[0]["numNodes"]
+        self.num_cases = 1 + (len(extra_loads) if fix is not None else 0)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -119,7 +130,8 @@ class Oracle:
 
     # ------------------------------------------------------------- accessors
     _types = {"transferMatCoeffi": np.float64, "diagK": np.float64, "Ks": np.float64, "RI": np.float64,
-              "Ke": np.float64, "F": np.float64, "KesFree": np.float64, "KesFixed": np.float64,
+              "Ke": np.float64, "F": np.float64, "objWeight": np.float64, "KesFree": np.float64,
+              "KesFixed": np.float64,
               "filterKernel0": np.float64, "filterDiag0": np.float64, "fixState": np.uint8}
 
     def get(self, name: str, level: int = 0) -> np.ndarray:
@@ -141,6 +153,21 @@ class Oracle:
     def apply_K(self, u):
         y = np.zeros(3 * self.nN)
         self.L.orc_apply_K(self.h, np.ascontiguousarray(u, dtype=np.float64), y)
+        return y
+
+    def set_vcycle(self, kind):
+        """typeVcycle_ (34): 'Adapted' or 'Standard'."""
+        if self.L.orc_set_vcycle(self.h, {"Adapted": 0, "Standard": 1}[kind]) != 0:
+            raise ValueError(kind)
+
+    def set_omega(self, w):
+        """weightFactorJacobi_ (32)."""
+        self.L.orc_set_omega(self.h, float(w))
+
+    def apply_K_level(self, level, u):
+        y = np.zeros(3 * self.levels[level]["numNodes"])
+        if self.L.orc_apply_K_level(self.h, level, np.ascontiguousarray(u, dtype=np.float64), y) != 0:
+            raise ValueError(level)
         return y
 
     def vcycle(self, r):
@@ -207,9 +234,10 @@ class Oracle:
         nE = self.nE
         xPhys = np.zeros(nE)
         cH, vH, conH, sH = (np.zeros(max(nLoop, 1)) for _ in range(4))
-        its = np.zeros(max(nLoop, 1), dtype=np.int32)
+        nc = self.num_cases
+        its = np.zeros(max(nLoop, 1) * nc, dtype=np.int32)
         fits = np.zeros(max(nLoop, 1), dtype=np.int32)
-        resH = np.zeros(max(nLoop, 1))
+        resH = np.zeros(max(nLoop, 1) * nc)
         tH = np.zeros(max(nLoop, 1) * 6)
         snap = np.zeros(max(nSnap, 1) * nE)
         cS = C.c_double(0.0)
@@ -221,7 +249,7 @@ class Oracle:
             raise RuntimeError(self.L.orc_last_error(self.h).decode())
         k = loops.value
         return dict(xPhys=xPhys, cHist=cH[:k], volHist=vH[:k], consHist=conH[:k], sharpHist=sH[:k],
-                    lssIts=its[:k], filterIts=fits[:k], resHist=resH[:k], tHist=tH[:6 * k].reshape(k, 6),
+                    lssIts=its[:k * nc], filterIts=fits[:k], resHist=resH[:k * nc], tHist=tH[:6 * k].reshape(k, 6),
                     xSnap=snap[:min(nSnap, k) * nE].reshape(-1, nE), cSolid=cS.value, loops=k,
                     warnings=self.L.orc_warnings(self.h))
 

@@ -69,6 +69,7 @@ struct Settings {
   double omega = 0.6;       // weightFactorJacobi_ 32
   int coarsestControl = 50000;  // 33
   int nonDyadic = 1;        // 35
+  int typeVcycle = 0;       // typeVcycle_ 34: 0 'Adapted', 1 'Standard'
 };
 
 // ------------------------------------------------------- deterministic sums
@@ -266,9 +267,11 @@ struct Model {
   // boundary conditions (compact finest node ids, 0-based)
   std::vector<int> fixNode;               // sorted unique
   std::vector<unsigned char> fixState;    // 3 per fixed node
-  std::vector<int> loadNode;
-  std::vector<double> loadVal;            // 3 per loaded node
-  std::vector<double> F;                  // 3*numNodes
+  // load cases (loadingCond_{ii}, 447, 481-493): nodes and 3 values per node
+  std::vector<std::vector<int>> loadNode;
+  std::vector<std::vector<double>> loadVal;
+  std::vector<std::vector<double>> F;     // F_ columns, 3*numNodes each (1315-1321)
+  std::vector<double> objWeight;          // objWeightingList_ (493, 558)
   std::vector<char> isPassive;            // per compact element (passiveElements_, 81 / 274)
   bool bcOk = false;
   // BC element matrices, 1338-1437
@@ -474,45 +477,48 @@ void cuboid_bc(Model& M, std::vector<int>& fixDense, std::vector<int>& loadDense
   std::sort(loadDense.begin(), loadDense.end());
 }
 
-// FEA_ApplyBoundaryCondition, 1294-1336 (single load case).
+// FEA_ApplyBoundaryCondition, 1294-1336, over all load cases.
 bool apply_bc(Model& M) {
-  // 1303-1306: loaded nodes are not fixed; unique+sort
-  std::vector<int> fn; std::vector<unsigned char> fs;
-  {
-    std::vector<std::pair<int, int>> tmp;
-    for (size_t i = 0; i < M.fixNode.size(); ++i) tmp.push_back({M.fixNode[i], (int)i});
-    std::sort(tmp.begin(), tmp.end());
-    std::vector<int> ln = M.loadNode; std::sort(ln.begin(), ln.end());
-    int last = -1;
-    for (auto& p : tmp) {
-      if (p.first == last) continue;
-      last = p.first;
-      if (std::binary_search(ln.begin(), ln.end(), p.first)) continue;
-      fn.push_back(p.first);
-      for (int c = 0; c < 3; ++c) fs.push_back(M.fixState[(size_t)p.second * 3 + c]);
+  for (size_t ii = 0; ii < M.loadNode.size(); ++ii) {  // 1301-1312
+    // 1303-1306: nodes loaded in this case are not fixed; unique+sort
+    std::vector<int> fn; std::vector<unsigned char> fs;
+    {
+      std::vector<std::pair<int, int>> tmp;
+      for (size_t i = 0; i < M.fixNode.size(); ++i) tmp.push_back({M.fixNode[i], (int)i});
+      std::sort(tmp.begin(), tmp.end());
+      std::vector<int> ln = M.loadNode[ii]; std::sort(ln.begin(), ln.end());
+      int last = -1;
+      for (auto& p : tmp) {
+        if (p.first == last) continue;
+        last = p.first;
+        if (std::binary_search(ln.begin(), ln.end(), p.first)) continue;
+        fn.push_back(p.first);
+        for (int c = 0; c < 3; ++c) fs.push_back(M.fixState[(size_t)p.second * 3 + c]);
+      }
     }
-  }
-  M.fixNode = fn; M.fixState = fs;
-  {  // 1307-1308: a node listed twice keeps its first load row
-    std::vector<std::pair<int, int>> tmp;
-    for (size_t i = 0; i < M.loadNode.size(); ++i) tmp.push_back({M.loadNode[i], (int)i});
-    std::sort(tmp.begin(), tmp.end());
-    std::vector<int> ln; std::vector<double> lv;
-    int last = -1;
-    for (auto& p : tmp) {
-      if (p.first == last) continue;
-      last = p.first;
-      ln.push_back(p.first);
-      for (int c = 0; c < 3; ++c) lv.push_back(M.loadVal[(size_t)p.second * 3 + c]);
+    M.fixNode = fn; M.fixState = fs;
+    {  // 1307-1308: a node listed twice keeps its first load row
+      std::vector<std::pair<int, int>> tmp;
+      for (size_t i = 0; i < M.loadNode[ii].size(); ++i) tmp.push_back({M.loadNode[ii][i], (int)i});
+      std::sort(tmp.begin(), tmp.end());
+      std::vector<int> ln; std::vector<double> lv;
+      int last = -1;
+      for (auto& p : tmp) {
+        if (p.first == last) continue;
+        last = p.first;
+        ln.push_back(p.first);
+        for (int c = 0; c < 3; ++c) lv.push_back(M.loadVal[ii][(size_t)p.second * 3 + c]);
+      }
+      M.loadNode[ii] = ln; M.loadVal[ii] = lv;
     }
-    M.loadNode = ln; M.loadVal = lv;
+    if (M.loadNode[ii].empty()) { M.err = "No Loads!"; return false; }    // 1309
+    if (M.fixNode.empty()) { M.err = "No Fixations!"; return false; }     // 1310
   }
-  if (M.loadNode.empty()) { M.err = "No Loads!"; return false; }        // 1309
-  if (M.fixNode.empty()) { M.err = "No Fixations!"; return false; }     // 1310
   const Level& L = M.L[0];
-  M.F.assign((size_t)L.numNodes * 3, 0.0);  // 1315-1321
-  for (size_t i = 0; i < M.loadNode.size(); ++i)
-    for (int c = 0; c < 3; ++c) M.F[(size_t)M.loadNode[i] * 3 + c] = M.loadVal[i * 3 + c];
+  M.F.assign(M.loadNode.size(), std::vector<double>((size_t)L.numNodes * 3, 0.0));  // 1315-1321
+  for (size_t ii = 0; ii < M.loadNode.size(); ++ii)
+    for (size_t i = 0; i < M.loadNode[ii].size(); ++i)
+      for (int c = 0; c < 3; ++c) M.F[ii][(size_t)M.loadNode[ii][i] * 3 + c] = M.loadVal[ii][i * 3 + c];
   return true;
 }
 
@@ -970,6 +976,79 @@ void vcycle(const Model& M, const double* r, double* x) {
   std::memcpy(x, X[0].data(), X[0].size() * sizeof(double));
 }
 
+// Solving_KbyU_MatrixFree for iLevel > 1 (727-744): per-element products with
+// the stored Galerkin matrices Ks, accumulated per node (accumarray 736-741).
+void apply_K_level(const Model& M, int level, const double* u, double* Y) {
+  const Level& L = M.L[level];
+  std::vector<double> prod((size_t)L.numElements * 24);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < L.numElements; ++e) {
+    double ue[24];
+    for (int j = 0; j < 8; ++j)
+      for (int c = 0; c < 3; ++c) ue[3 * j + c] = u[(size_t)L.eNodMat[(size_t)e * 8 + j] * 3 + c];
+    const double* K = &L.Ks[(size_t)e * 576];  // column-major 24x24
+    for (int k = 0; k < 24; ++k) {  // 734: (u_e * Ks)(k) = sum_b u_e(b) Ks(b, k)
+      double s = 0.0;
+      for (int b = 0; b < 24; ++b) s += ue[b] * K[k * 24 + b];
+      prod[(size_t)e * 24 + k] = s;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int n = 0; n < L.numNodes; ++n) {
+    double s[3] = {0, 0, 0};
+    for (int t = L.n2eStart[n]; t < L.n2eStart[n + 1]; ++t) {
+      int e = L.n2eList[t] / 8, j = L.n2eList[t] % 8;
+      for (int c = 0; c < 3; ++c) s[c] += prod[(size_t)e * 24 + 3 * j + c];
+    }
+    for (int c = 0; c < 3; ++c) Y[(size_t)n * 3 + c] = s[c];
+  }
+}
+
+// K*u on any level: the fine operator with E and the BC rows on level 0,
+// the stored Galerkin matrices above it.
+void apply_K_any(const Model& M, int level, const double* u, double* Y) {
+  if (level == 0) apply_K(M, u, Y);
+  else apply_K_level(M, level, u, Y);
+}
+
+// Solving_Vcycle ('Standard'), 747-781: damped-Jacobi pre-smoothing from
+// zero, defect restriction (759-762), coarsest solve (767), coarse-grid
+// correction (770) and one damped-Jacobi post-smoothing step (775-776).
+void vcycle_standard(const Model& M, const double* r, double* x) {
+  const int NL = M.numLevels;
+  const double w = M.S.omega;
+  std::vector<std::vector<double>> R(NL), X(NL);
+  R[0].assign(r, r + (size_t)M.L[0].numNodes * 3);
+  for (int ii = 0; ii < NL - 1; ++ii) {
+    const Level& L = M.L[ii];
+    const size_t n = (size_t)L.numNodes * 3;
+    X[ii].resize(n);
+    for (size_t k = 0; k < n; ++k) X[ii][k] = w * (R[ii][k] / L.diagK[k]);  // 758
+    std::vector<double> d(n);
+    apply_K_any(M, ii, X[ii].data(), d.data());
+    for (size_t k = 0; k < n; ++k) d[k] = R[ii][k] - d[k];  // 764
+    R[ii + 1].resize((size_t)M.L[ii + 1].numNodes * 3);
+    restrict_residual(M.L[ii + 1], d.data(), R[ii + 1].data());  // 765
+  }
+  X[NL - 1].resize(R[NL - 1].size());
+  coarse_solve(M, R[NL - 1].data(), X[NL - 1].data());  // 767
+  for (int ii = NL - 2; ii >= 0; --ii) {
+    const Level& L = M.L[ii];
+    const size_t n = (size_t)L.numNodes * 3;
+    std::vector<double> px(n), kx(n);
+    interpolate(M.L[ii + 1], X[ii + 1].data(), px.data());  // 770
+    for (size_t k = 0; k < n; ++k) X[ii][k] += px[k];
+    apply_K_any(M, ii, X[ii].data(), kx.data());
+    for (size_t k = 0; k < n; ++k) X[ii][k] += w * (R[ii][k] - kx[k]) / L.diagK[k];  // 775-776
+  }
+  std::memcpy(x, X[0].data(), X[0].size() * sizeof(double));
+}
+
+void precondition(const Model& M, const double* r, double* x) {
+  if (M.S.typeVcycle == 1) vcycle_standard(M, r, x);
+  else vcycle(M, r, x);
+}
+
 // ------------------------------------------------------------------- PCG
 // Solving_PCG, 600-639.  Returns its; *res = final relative residual.
 template <class AOp, class POp>
@@ -1009,7 +1088,8 @@ int pcg(AOp A, POp P, const std::vector<double>& b, std::vector<double>& y, doub
 }
 
 // ---------------------------------------------------------------- compliance
-// TopOpti_ComputeUnitCompliance, 579-598 (objWeightingList_ = 1).
+// TopOpti_ComputeUnitCompliance, 579-598, one load case (the columns of
+// ceList, 594).
 void compliance(const Model& M, const double* U, double* ce) {
   const Level& L = M.L[0];
 #pragma omp parallel for schedule(static)
@@ -1024,6 +1104,17 @@ void compliance(const Model& M, const double* U, double* ce) {
       s += t * ue[k];
     }
     ce[e] = s;
+  }
+}
+
+// ceList * objWeightingList_(:) (597): the weighted sum over the load cases.
+void compliance_cases(const Model& M, const std::vector<std::vector<double>>& U, double* ce) {
+  const int nE = M.L[0].numElements;
+  std::vector<double> cj(nE);
+  for (int e = 0; e < nE; ++e) ce[e] = 0.0;
+  for (size_t j = 0; j < U.size(); ++j) {
+    compliance(M, U[j].data(), cj.data());
+    for (int e = 0; e < nE; ++e) ce[e] += cj[e] * M.objWeight[j];
   }
 }
 
@@ -1118,10 +1209,10 @@ int filter_run(const Model& M, const Filter& Fl, const double* src, double* tar)
 }
 
 // ------------------------------------------------------------------ solver
-int solve_fea(Model& M, std::vector<double>& U, bool* warned, double* res) {
+int solve_fea(Model& M, int lc, std::vector<double>& U, bool* warned, double* res) {
   auto A = [&](const double* u, double* y) { apply_K(M, u, y); };
-  auto P = [&](const double* r, double* z) { vcycle(M, r, z); };
-  return pcg(A, P, M.F, U, M.S.tol, M.S.maxIT, res, warned);
+  auto P = [&](const double* r, double* z) { precondition(M, r, z); };
+  return pcg(A, P, M.F[lc], U, M.S.tol, M.S.maxIT, res, warned);
 }
 
 // --------------------------------------------------------------------- OC
@@ -1306,14 +1397,14 @@ struct RunOut {
   double* volHist = nullptr;
   double* consHist = nullptr;
   double* sharpHist = nullptr;
-  int* lssIts = nullptr;       // nLoop
+  int* lssIts = nullptr;       // nLoop*nCases (lssIts(end+1), 121: one entry per solve)
   double* tHist = nullptr;     // nLoop*6
   double* xSnap = nullptr;     // nSnap*numElements
   int nSnap = 0;
   double* cSolid = nullptr;
   int* loops = nullptr;
   int* filterIts = nullptr;    // nLoop (sum of filter CG iterations)
-  double* resHist = nullptr;   // nLoop: final relative residual of each elasticity solve
+  double* resHist = nullptr;   // nLoop*nCases: final relative residual of each elasticity solve
 };
 
 inline double pow16(double v) { double v2 = v * v, v4 = v2 * v2, v8 = v4 * v4; return v8 * v8; }
@@ -1326,14 +1417,18 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
   setup_filter(M, rMin, M.filt[0]);
   if (mode == 1) setup_filter(M, rHat, M.filt[1]);
   std::vector<double> x(nE, V0), xPhys(nE, V0), xTilde(nE, V0), volMax(nE, V0);
-  std::vector<double> U((size_t)L.numNodes * 3, 0.0), ce(nE), dc(nE), dv(nE, 1.0);
+  const int nCases = (int)M.F.size();
+  std::vector<std::vector<double>> U(nCases, std::vector<double>((size_t)L.numNodes * 3, 0.0));  // 95 / 295
+  std::vector<double> ce(nE), dc(nE), dv(nE, 1.0);
   // solid reference solve (95-106 / 295-306)
   M.eleModulus.assign(nE, E0);
   if (!assemble(M)) return -1;
   bool warned = false;
-  solve_fea(M, U, &warned, nullptr);
-  if (warned) M.warnings++;
-  compliance(M, U.data(), ce.data());
+  for (int lc = 0; lc < nCases; ++lc) {  // 101 / 301
+    solve_fea(M, lc, U[lc], &warned, nullptr);
+    if (warned) M.warnings++;
+  }
+  compliance_cases(M, U, ce.data());
   const double cSolid = det_sum(nE, [&](i64 i) { return M.eleModulus[i] * ce[i]; });
   if (O.cSolid) *O.cSolid = cSolid;
   auto SIMP = [&](double xp) { return Emin + (xp * xp * xp) * (E0 - Emin); };       // 107
@@ -1350,11 +1445,15 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
     const double ta = now_s();
     if (!assemble(M)) return -1;  // 118
     const double tb = now_s();
-    double resFinal = 0.0;
-    int its = solve_fea(M, U, &warned, &resFinal);  // 121
-    if (warned) M.warnings++;
+    for (int lc = 0; lc < nCases; ++lc) {  // 121 / 321: one warm-started solve per F_ column
+      double resFinal = 0.0;
+      const int its = solve_fea(M, lc, U[lc], &warned, &resFinal);
+      if (warned) M.warnings++;
+      if (O.lssIts) O.lssIts[(size_t)(loop - 1) * nCases + lc] = its;
+      if (O.resHist) O.resHist[(size_t)(loop - 1) * nCases + lc] = resFinal;
+    }
     const double tc = now_s();
-    compliance(M, U.data(), ce.data());  // 124
+    compliance_cases(M, U, ce.data());  // 124
     double cObj = det_sum(nE, [&](i64 i) { return M.eleModulus[i] * (ce[i] / cSolid); });  // 125-126
     const double cDesign = cObj * cSolid;  // 127
     const double V = det_sum(nE, [&](i64 i) { return xPhys[i]; }) / (double)nE;  // 128
@@ -1435,9 +1534,7 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
     if (O.volHist) O.volHist[k] = V;
     if (O.consHist) O.consHist[k] = fval;
     if (O.sharpHist) O.sharpHist[k] = sharp;
-    if (O.lssIts) O.lssIts[k] = its;
     if (O.filterIts) O.filterIts[k] = fIts;
-    if (O.resHist) O.resHist[k] = resFinal;
     if (O.tHist) {
       double* th = &O.tHist[(size_t)k * 6];
       th[0] = tb - ta; th[1] = tc - tb; th[2] = tOpt; th[3] = tFilt; th[4] = tLVF; th[5] = tTot;
@@ -1455,25 +1552,35 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
 extern "C" {
 
 // Builds the model.  voxels: uint8 nely*nelx*nelz column-major (y fastest).
-// nFix < 0 selects the built-in cuboid BCs (536-559).  Otherwise fixNodes /
-// loadNodes are 0-based node ids of the UNPADDED grid (nely+1, nelx+1,
-// nelz+1), column-major; fixState is nFix*3 flags; loadVec nLoad*3 values.
-void* orc_create(const unsigned char* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
-                 const unsigned char* fixState, int nLoad, const int* loadNodes, const double* loadVec) {
+// nFix < 0 selects the built-in cuboid BCs (536-559, one load case, weight
+// 1).  Otherwise fixNodes / loadNodes are 0-based node ids of the UNPADDED
+// grid (nely+1, nelx+1, nelz+1), column-major; fixState is nFix*3 flags;
+// the nCases load cases (loadingCond_, 481-493) are concatenated, case ii
+// holding nPerCase[ii] nodes and 3 values per node; objective weights
+// 1/nCases (493).
+void* orc_create_multi(const unsigned char* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                       const unsigned char* fixState, int nCases, const int* nPerCase, const int* loadNodes,
+                       const double* loadVec) {
   Model* M = new Model();
   M->nely = nely; M->nelx = nelx; M->nelz = nelz;
   if (nelx < 3 || nely < 3 || nelz < 3) { M->err = "Inappropriate Input Model!"; return M; }  // 532-534
   discretize(*M, voxels);
   M->Ke = element_stiffness(M->S);
   const Level& L0 = M->L[0];
-  std::vector<int> fixDense, loadDense;
-  std::vector<double> lv;
+  std::vector<int> fixDense;
+  std::vector<std::vector<int>> loadDense;
+  std::vector<std::vector<double>> lv;
   std::vector<unsigned char> fs;
   if (nFix < 0) {
-    cuboid_bc(*M, fixDense, loadDense);
+    loadDense.resize(1);
+    lv.resize(1);
+    cuboid_bc(*M, fixDense, loadDense[0]);
     fs.assign(fixDense.size() * 3, 1);
-    const double q = 1.0 / (double)loadDense.size();  // 556
-    for (size_t i = 0; i < loadDense.size(); ++i) { lv.push_back(0.0 * q); lv.push_back(0.0 * q); lv.push_back(-1.0 * q); }
+    const double q = 1.0 / (double)loadDense[0].size();  // 556
+    for (size_t i = 0; i < loadDense[0].size(); ++i) {
+      lv[0].push_back(0.0 * q); lv[0].push_back(0.0 * q); lv[0].push_back(-1.0 * q);
+    }
+    M->objWeight.assign(1, 1.0);  // 558
   } else {
     // AdaptBCExternalMdl (1439-1489): unpadded -> padded node ids
     const i64 uy = nely + 1, ux = nelx + 1;
@@ -1483,7 +1590,16 @@ void* orc_create(const unsigned char* voxels, int nely, int nelx, int nelz, int 
       return (int)(jy + PY * (jx + PX * jz));
     };
     for (int i = 0; i < nFix; ++i) { fixDense.push_back(adapt(fixNodes[i])); for (int c = 0; c < 3; ++c) fs.push_back(fixState[i * 3 + c]); }
-    for (int i = 0; i < nLoad; ++i) { loadDense.push_back(adapt(loadNodes[i])); for (int c = 0; c < 3; ++c) lv.push_back(loadVec[i * 3 + c]); }
+    if (nCases < 1) { M->err = "No Loads!"; return M; }
+    loadDense.resize(nCases);
+    lv.resize(nCases);
+    int off = 0;
+    for (int ii = 0; ii < nCases; ++ii)
+      for (int i = 0; i < nPerCase[ii]; ++i, ++off) {
+        loadDense[ii].push_back(adapt(loadNodes[off]));
+        for (int c = 0; c < 3; ++c) lv[ii].push_back(loadVec[(size_t)off * 3 + c]);
+      }
+    M->objWeight.assign(nCases, 1.0 / nCases);  // 493
   }
   for (size_t i = 0; i < fixDense.size(); ++i) {
     int k = L0.nodMapForward[fixDense[i]] - 1;
@@ -1491,17 +1607,26 @@ void* orc_create(const unsigned char* voxels, int nely, int nelx, int nelz, int 
     M->fixNode.push_back(k);
     for (int c = 0; c < 3; ++c) M->fixState.push_back(fs[i * 3 + c]);
   }
-  for (size_t i = 0; i < loadDense.size(); ++i) {
-    int k = L0.nodMapForward[loadDense[i]] - 1;
-    if (k < 0) continue;
-    M->loadNode.push_back(k);
-    for (int c = 0; c < 3; ++c) M->loadVal.push_back(lv[i * 3 + c]);
-  }
+  M->loadNode.assign(loadDense.size(), {});
+  M->loadVal.assign(loadDense.size(), {});
+  for (size_t ii = 0; ii < loadDense.size(); ++ii)
+    for (size_t i = 0; i < loadDense[ii].size(); ++i) {
+      int k = L0.nodMapForward[loadDense[ii][i]] - 1;
+      if (k < 0) continue;
+      M->loadNode[ii].push_back(k);
+      for (int c = 0; c < 3; ++c) M->loadVal[ii].push_back(lv[ii][i * 3 + c]);
+    }
   if (!apply_bc(*M)) return M;
   build_hierarchy(*M);
   setup_bc_ke(*M);
   M->bcOk = true;
   return M;
+}
+
+// One load case.
+void* orc_create(const unsigned char* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                 const unsigned char* fixState, int nLoad, const int* loadNodes, const double* loadVec) {
+  return orc_create_multi(voxels, nely, nelx, nelz, nFix, fixNodes, fixState, 1, &nLoad, loadNodes, loadVec);
 }
 
 void orc_destroy(void* h) { delete (Model*)h; }
@@ -1544,7 +1669,11 @@ long long orc_get(void* h, const char* name, int level, void* out, long long cap
   if (n == "Ks") return cp(L.Ks.data(), L.Ks.size(), sizeof(double));
   if (n == "RI") return cp(L.RI.data(), L.RI.size(), sizeof(double));
   if (n == "Ke") return cp(M->Ke.data(), M->Ke.size(), sizeof(double));
-  if (n == "F") return cp(M->F.data(), M->F.size(), sizeof(double));
+  if (n == "F") {  // level: the load case
+    if (level < 0 || level >= (int)M->F.size()) return -1;
+    return cp(M->F[level].data(), M->F[level].size(), sizeof(double));
+  }
+  if (n == "objWeight") return cp(M->objWeight.data(), M->objWeight.size(), sizeof(double));
   if (n == "fixNode") return cp(M->fixNode.data(), M->fixNode.size(), sizeof(int));
   if (n == "fixState") return cp(M->fixState.data(), M->fixState.size(), 1);
   if (n == "mapUniqueKes") return cp(M->mapUniqueKes.data(), M->mapUniqueKes.size(), sizeof(int));
@@ -1563,7 +1692,26 @@ int orc_set_modulus(void* h, const double* E) {
 }
 int orc_assemble(void* h) { return assemble(*(Model*)h) ? 0 : -1; }
 int orc_apply_K(void* h, const double* u, double* y) { apply_K(*(Model*)h, u, y); return 0; }
-int orc_vcycle(void* h, const double* r, double* x) { vcycle(*(Model*)h, r, x); return 0; }
+int orc_vcycle(void* h, const double* r, double* x) { precondition(*(Model*)h, r, x); return 0; }
+// typeVcycle_ (34): 0 'Adapted', 1 'Standard'.  Ks is kept on every coarse
+// level either way (997-999 keeps it for 'Standard').
+int orc_set_vcycle(void* h, int type) {
+  if (type != 0 && type != 1) return 1;
+  ((Model*)h)->S.typeVcycle = type;
+  return 0;
+}
+// weightFactorJacobi_ (32).
+int orc_set_omega(void* h, double w) {
+  ((Model*)h)->S.omega = w;
+  return 0;
+}
+// K*u on one level (0: 641-726, >0: 727-744), compact node order.
+int orc_apply_K_level(void* h, int level, const double* u, double* y) {
+  Model* M = (Model*)h;
+  if (level < 0 || level >= M->numLevels) return 1;
+  apply_K_any(*M, level, u, y);
+  return 0;
+}
 int orc_coarse_solve(void* h, const double* r, double* x) { coarse_solve(*(Model*)h, r, x); return 0; }
 int orc_restrict(void* h, int level, const double* rF, double* rC) {
   restrict_residual(((Model*)h)->L[level], rF, rC); return 0;
@@ -1577,7 +1725,7 @@ int orc_pcg(void* h, const double* b, double* x, double tol, int maxIT, double* 
   const size_t n = (size_t)M->L[0].numNodes * 3;
   std::vector<double> bb(b, b + n), y(x, x + n);
   auto A = [&](const double* u, double* yy) { apply_K(*M, u, yy); };
-  auto P = [&](const double* r, double* z) { vcycle(*M, r, z); };
+  auto P = [&](const double* r, double* z) { precondition(*M, r, z); };
   int its = pcg(A, P, bb, y, tol, maxIT, res, nullptr);
   std::memcpy(x, y.data(), n * sizeof(double));
   return its;

@@ -244,6 +244,21 @@ class Grid:
             Minv = np.diag(2 * omega / d) + Pl @ Minv @ Pl.T
         return Minv
 
+    def standard_vcycle_matrix(self, E, omega=0.6):
+        """'Standard' V(1,1) operator from its error propagation: with
+        S = I - w D^-1 A, M_l = (I - S (I - P M_(l+1) P^T A) S) A^-1."""
+        A, P = self.hierarchy(E)
+        L = self.nlev
+        M = np.linalg.inv(A[L - 1].toarray())
+        for l in range(L - 2, -1, -1):
+            Al = A[l].toarray()
+            n = Al.shape[0]
+            S = np.eye(n) - omega * (1.0 / np.diag(Al))[:, None] * Al
+            Pl = P[l + 1].toarray()
+            Err = S @ (np.eye(n) - Pl @ M @ Pl.T @ Al) @ S
+            M = (np.eye(n) - Err) @ np.linalg.inv(Al)
+        return M
+
 
 def pcg_numpy(A, M, b, x0, tol, maxit):
     """Restated PCG (600-639) with dense/sparse operators."""

@@ -196,16 +196,17 @@ def test_run_local_small(oracle_lib, name):
     assert np.abs(rg["xSnap"] - ro["xSnap"]).max() < SMALL_TOL
 
 
-def _trajectory_parity(mode, V0, nLoop, rHat, n_strict):
+def _trajectory_parity(mode, V0, nLoop, rHat, n_strict, vcycle="Adapted"):
     """GPU vs oracle trajectory on true(50,100,50), widened to 5x the FP64 noise
     floor measured live from three oracle coarsest-level orderings."""
     import top3d_xl_amd as t3x
     vox = np.ones((50, 100, 50), bool)
-    print(f"\n[gpu-parity] 50x100x50 {mode} {nLoop} iterations", flush=True)
-    rg = t3x.Problem(vox).run(mode, V0, nLoop, math.sqrt(3), rHat, nSnap=nLoop)
+    print(f"\n[gpu-parity] 50x100x50 {mode} {vcycle} {nLoop} iterations", flush=True)
+    rg = t3x.Problem(vox, vcycle=int(vcycle == "Standard")).run(mode, V0, nLoop, math.sqrt(3), rHat, nSnap=nLoop)
     ros = []
     for v in (0, 1, 2):
         o = Oracle(vox)
+        o.set_vcycle(vcycle)
         o.set_order_variant(v)
         ros.append(o.run(mode, V0, nLoop, math.sqrt(3), rHat, nSnap=nLoop))
     ro = ros[0]

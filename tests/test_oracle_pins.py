@@ -228,6 +228,68 @@ def test_vcycle_is_the_dense_operator(case):
     assert np.linalg.eigvalsh(S).min() > 0
 
 
+def test_level_K_is_the_galerkin_operator(case):
+    """K*u with the stored Ks (727-744) on every level equals P^T K_BC P built
+    from scipy-assembled fine K and hat-function prolongations; level 0 is
+    the BC operator itself."""
+    o, g, E, rng = case["o"], case["g"], case["E"], case["rng"]
+    A, _ = g.hierarchy(E)
+    for l in range(g.nlev):
+        u = rng.standard_normal(3 * g.nN[l])
+        y = o.apply_K_level(l, u)
+        yr = A[l] @ u
+        np.testing.assert_allclose(y, yr, rtol=0, atol=1e-12 * np.abs(yr).max())
+
+
+def test_standard_vcycle_error_propagation(case):
+    """'Standard' V-cycle (759-777) against its error-propagation form: with
+    S = I - w D^-1 A, one level maps the error by S (I - P M_c P^T A) S, so
+    M = (I - S (I - P M_c P^T A) S) A^-1, recursively from M_L = A_L^-1.
+    A dropped smoothing step, a sign error in the defect or a missing
+    coarse correction changes this operator.  M is symmetric; unlike the
+    'Adapted' operator it need not be positive definite: w = 0.6 is above
+    2 / lambda_max(D^-1 A) wherever that eigenvalue exceeds 3.33 (measured
+    3.09 on the uniform 6x12x6 fine level but 3.40 on its level 2 and 4.11
+    with the 1e-6..1 contrast field), so the Jacobi step is no contraction in
+    the energy norm there."""
+    o, g, E, rng = case["o"], case["g"], case["E"], case["rng"]
+    M = g.standard_vcycle_matrix(E)
+    o.set_vcycle("Standard")
+    try:
+        for _ in range(2):
+            r = rng.standard_normal(3 * o.nN)
+            z = o.vcycle(r)
+            zr = M @ r
+            np.testing.assert_allclose(z, zr, rtol=0, atol=1e-9 * np.abs(zr).max())
+    finally:
+        o.set_vcycle("Adapted")
+    assert np.abs(M - M.T).max() < 1e-8 * np.abs(M).max()
+    # and it differs from the 'Adapted' operator (the two types are not confused)
+    Ma = g.vcycle_matrix(E)
+    assert np.abs(M - Ma).max() > 1e-3 * np.abs(Ma).max()
+
+
+def test_standard_pcg_matches_numpy_restatement(case):
+    """PCG (600-639) preconditioned by the 'Standard' cycle against the numpy
+    PCG with the dense operator of its error-propagation form."""
+    o, g = case["o"], case["g"]
+    E = np.ones(o.nE)
+    o.set_modulus(E)
+    o.assemble()
+    A, _ = g.hierarchy(E)
+    M = g.standard_vcycle_matrix(E)
+    o.set_vcycle("Standard")
+    try:
+        x, its, _ = o.pcg(g.F, tol=1e-3, maxit=800)
+    finally:
+        o.set_vcycle("Adapted")
+        o.set_modulus(case["E"])
+        o.assemble()
+    xr, itr = bf.pcg_numpy(A[0], lambda r: M @ r, g.F, np.zeros_like(g.F), 1e-3, 800)
+    assert its == itr
+    np.testing.assert_allclose(x, xr, rtol=0, atol=1e-7 * np.abs(xr).max())
+
+
 # ---------------------------------------------------------- PCG (a4)
 @pytest.mark.parametrize("field", ["uniform", "rough"])
 def test_pcg_matches_numpy_restatement(case, field):

@@ -5,3 +5,4 @@ The compute lives in ``libt3x.so`` (C ABI declared in ``include/t3x.h``);
 """
 from .t3x import (TOP3D_XL, Problem, Server, T3XError, load_library, default_params, footprint,  # noqa: F401
                   stream_copy_ms, EXPORTED)
+from . import topvoxel  # noqa: F401,E402

@@ -202,10 +202,11 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, const T* __restri
 // The scratch (24 floats per node slot) costs 8% of the matrix bytes.
 // Coarse levels carry no fixed DOFs (the boundary conditions are inside the
 // Galerkin matrices); vectors and matrices are zero on void nodes and cells.
-// FP32 throughout (the 'Standard' V-cycle smoother).
-__global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const float* __restrict__ Ks,
-                                                      const float* __restrict__ u, float* __restrict__ Yc,
-                                                      const int* fl) {
+// T = float (the mixed form's FP32 smoother) or double (the FP64 'Standard'
+// cycle).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const T* __restrict__ Ks, const T* __restrict__ u,
+                                                      T* __restrict__ Yc, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int plane = g.NY * g.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -215,21 +216,21 @@ __global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const float* __rest
   if (cy >= g.ny || cx >= g.nx || cz >= g.nz) return;
   const long long NN = g.NN;
   const long long cell = idx + (long long)plane * cz;
-  float ue[24], ye[24];
+  T ue[24], ye[24];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     const long long m = cell + kLocDy[a] + (long long)g.NY * kLocDx[a] + (long long)plane * kLocDz[a];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       ue[3 * a + c] = u[c * NN + m];
-      ye[3 * a + c] = 0.f;
+      ye[3 * a + c] = T(0);
     }
   }
 #pragma unroll
   for (int q = 0; q < 24; ++q) {
 #pragma unroll
     for (int p = 0; p <= q; ++p) {
-      const float v = Ks[(long long)(q * (q + 1) / 2 + p) * NN + cell];
+      const T v = Ks[(long long)(q * (q + 1) / 2 + p) * NN + cell];
       ye[p] += v * ue[q];
       if (p != q) ye[q] += v * ue[p];
     }
@@ -238,12 +239,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const float* __rest
   for (int i = 0; i < 24; ++i) Yc[(long long)i * NN + cell] = ye[i];
 }
 
-template <int EPI>
-__global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const float* __restrict__ Yc,
-                                                           const float* __restrict__ u, float* __restrict__ y,
-                                                           const float* __restrict__ rv,
-                                                           const float* __restrict__ dv, double omega,
-                                                           const int* fl) {
+template <typename T, int EPI>
+__global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const T* __restrict__ Yc, const T* __restrict__ u,
+                                                           T* __restrict__ y, const T* __restrict__ rv,
+                                                           const T* __restrict__ dv, double omega, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int plane = g.NY * g.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -252,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const float* _
   const int z = blockIdx.y;
   const long long NN = g.NN;
   const long long n = idx + (long long)plane * z;
-  float acc[3] = {0.f, 0.f, 0.f};
+  T acc[3] = {0, 0, 0};
 #pragma unroll
   for (int cc = 0; cc < 8; ++cc) {
     const int CY = cc & 1, CX = (cc >> 1) & 1, CZ = cc >> 2;
@@ -268,8 +267,8 @@ __global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const float* _
     if (EPI == EPI_RESID) {
       y[c * NN + n] = rv[c * NN + n] - acc[c];
     } else {
-      const float dc = dv[c * NN + n];
-      y[c * NN + n] = dc != 0.f ? u[c * NN + n] + float(omega) * (rv[c * NN + n] - acc[c]) / dc : 0.f;
+      const T dc = dv[c * NN + n];
+      y[c * NN + n] = dc != T(0) ? u[c * NN + n] + T(omega) * (rv[c * NN + n] - acc[c]) / dc : T(0);
     }
   }
 }
@@ -294,18 +293,51 @@ static dim3 node_grid(const Geo& g) { return dim3((g.NY * g.NX + kBlock - 1) / k
 
 long long coarse_ks_tmp_len(const Geo& g) { return 24LL * g.NN; }
 
-void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
-                           float* tmp, const int* fl) {
-  hipLaunchKernelGGL(k_cell_Ks_u, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
-  hipLaunchKernelGGL(k_node_gather_Ks<EPI_RESID>, node_grid(g), dim3(kBlock), 0, s, g, (const float*)tmp, x, d, r,
-                     (const float*)nullptr, 0.0, fl);
+template <typename T>
+static void coarse_resid(hipStream_t s, const Geo& g, const T* Ks, const T* x, const T* r, T* d, T* tmp,
+                         const int* fl) {
+  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
+  hipLaunchKernelGGL((k_node_gather_Ks<T, EPI_RESID>), node_grid(g), dim3(kBlock), 0, s, g, (const T*)tmp, x, d, r,
+                     (const T*)nullptr, 0.0, fl);
 }
 
+template <typename T>
+static void coarse_smooth(hipStream_t s, const Geo& g, const T* Ks, const T* x, const T* r, const T* diag,
+                          double omega, T* out, T* tmp, const int* fl) {
+  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
+  hipLaunchKernelGGL((k_node_gather_Ks<T, EPI_SMOOTH>), node_grid(g), dim3(kBlock), 0, s, g, (const T*)tmp, x, out,
+                     r, diag, omega, fl);
+}
+
+void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
+                           float* tmp, const int* fl) {
+  coarse_resid(s, g, Ks, x, r, d, tmp, fl);
+}
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
                             const float* diag, double omega, float* out, float* tmp, const int* fl) {
-  hipLaunchKernelGGL(k_cell_Ks_u, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
-  hipLaunchKernelGGL(k_node_gather_Ks<EPI_SMOOTH>, node_grid(g), dim3(kBlock), 0, s, g, (const float*)tmp, x, out,
-                     r, diag, omega, fl);
+  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl);
+}
+void launch_coarse_resid64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
+                           double* d, double* tmp, const int* fl) {
+  coarse_resid(s, g, Ks, x, r, d, tmp, fl);
+}
+void launch_coarse_smooth64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
+                            const double* diag, double omega, double* out, double* tmp, const int* fl) {
+  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl);
+}
+
+// FP64 fine-level smoother products of the 'Standard' cycle (761, 775) with
+// E*Ke and the boundary-condition rows, through the tiled K*u.
+void launch_fine_resid64(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                         const double* x, const double* r, double* d, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<double, double, double, double, EPI_RESID, double, double>), applyk_grid(g),
+                     dim3(AK_T), 0, s, g, Ke, E, fix, x, d, (double*)nullptr, fl, 0, r, (const double*)nullptr, 0.0);
+}
+void launch_fine_smooth64(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                          const double* x, const double* r, const double* diag, double omega, double* out,
+                          double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_tiled<double, double, double, double, EPI_SMOOTH, double, double>), applyk_grid(g),
+                     dim3(AK_T), 0, s, g, Ke, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag, omega);
 }
 
 }  // namespace t3x

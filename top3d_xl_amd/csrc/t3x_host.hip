@@ -140,6 +140,8 @@ struct OptState {
   double V0 = 0.0, rMin = 0.0, rHat = 0.0, cSolid = 0.0;
   double change = 1.0, sharp = 1.0, beta = 1.0;  // 85-86, 233, 284-285
   std::vector<t3x_iter> hist;
+  std::vector<int> lssIts;      // one entry per elasticity solve (lssIts(end+1), 121)
+  std::vector<double> resAll;   // final relative residual of each solve
 };
 
 // HIP-event timing of kernel groups on the problem's stream.
@@ -170,11 +172,19 @@ struct t3x_problem {
   int nPassive = 0;
   double* bvec = nullptr;
   double *F = nullptr, *U = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *tmp3 = nullptr;
+  // load cases (loadingCond_, 481-493): F_ columns and their solutions; F/U
+  // above are case 0; objective weights objWeightingList_ (493, 558)
+  int nCases = 1;
+  std::vector<double*> Fc, Uc;
+  std::vector<double> wObj;
+  double* ceTmp = nullptr;  // one case's element compliance (nCases > 1)
   double *E = nullptr, *x = nullptr, *xPhys = nullptr, *ce = nullptr, *dc = nullptr;
   // mixed precision (precision == 1): FP32 V-cycle output; the 'Standard'
   // cycle adds FP32 Ke/E and the fine-level smoother vectors
   float* z32 = nullptr;
   float *Ke32 = nullptr, *E32 = nullptr, *x32 = nullptr, *d32 = nullptr;
+  // FP64 'Standard' cycle: fine-level smoother state and defect
+  double *x64 = nullptr, *d64 = nullptr;
   void* rtmp = nullptr;  // restriction intermediate / coarse smoother products (largest use)
   double *ca = nullptr, *cb = nullptr, *cc = nullptr, *cd = nullptr;  // cell scratch
   // filters (node scalars)
@@ -268,7 +278,7 @@ struct t3x_problem {
   long long NN() const { return lv[0].NN; }
   bool mixed() const { return P.precision == 1; }  // FP32 V-cycle
   const uint8_t* design_mask() const { return active ? active : lv[0].emask; }
-  bool standard() const { return P.vcycle == 1; }  // 'Standard' V-cycle (FP32)
+  bool standard() const { return P.vcycle == 1; }  // 'Standard' V-cycle (FP64 or, mixed, FP32)
   void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
   // Host-to-device upload ordered on the problem stream (a plain hipMemcpy from
   // pageable memory may return before its DMA lands and is not ordered with a
@@ -360,7 +370,8 @@ void build_levels(t3x_problem& P, const uint8_t* vox) {
 // per-level arrays, the coarsest band with the bandwidth of a full box in
 // its best axis order (a sub-domain's ordering is never wider), and 4 KiB of
 // allocation slack per array.
-long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t3x_params& prm, int mode) {
+long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t3x_params& prm, int mode,
+                             int nCases = 1) {
   const std::vector<LevelDims> d = level_dims(nely, nelx, nelz, nv, prm.coarsestControl);
   const int L = (int)d.size();
   const bool mixed = prm.precision == 1;
@@ -373,23 +384,26 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
     add(NN, 2);                                      // emask, nmask
     if (l < L - 1) add(24 * NN);                     // diag
     if (l >= 1) add(24 * NN, 2), add(8LL * kTri * NN);  // r, x, Ks
+    if (!mixed && prm.vcycle == 1 && l >= 1 && l < L - 1) add(24 * NN);  // t
     if (mixed) {
       if (l < L - 1) add(12 * NN);                   // diagf
       if (l >= 1) add(12 * NN, 2);                   // rf, xf
       if (prm.vcycle == 1 && l >= 1 && l < L - 1) add(12 * NN), add(4LL * kTri * NN);  // tf, Ksf
     }
     if (l + 1 < L) tmp = std::max(tmp, 3LL * (d[l + 1].ny + 1) * (d[l].nx + 1) * (d[l].nz + 1));
-    if (prm.vcycle == 1 && l >= 1 && l < L - 1) tmp = std::max(tmp, 12 * NN);  // coarse smoother products
+    if (prm.vcycle == 1 && l >= 1 && l < L - 1) tmp = std::max(tmp, (mixed ? 12 : 24) * NN);  // coarse smoother products
   }
   add(8 * tmp);                                      // restriction intermediate
   const long long NN = NNof(0);
   add(NN);                                           // fix
   add(24 * NN, 8);                                   // F, U, r, z, p, q, tmp3, bvec
+  if (nCases > 1) add(24 * NN, 2 * (nCases - 1)), add(8 * NN);  // further F_ columns and solutions, ceTmp
   add(8 * NN, 9);                                    // cell arrays
   add(8 * NN, 6);                                    // filter vectors
   add(8 * NN, 2);                                    // filter diagonals
   if (mixed) add(12 * NN);                           // z32
-  if (prm.vcycle == 1) add(576 * 4), add(4 * NN), add(12 * NN, 2);  // Ke32, E32, x32, d32
+  if (prm.vcycle == 1 && mixed) add(576 * 4), add(4 * NN), add(12 * NN, 2);  // Ke32, E32, x32, d32
+  if (prm.vcycle == 1 && !mixed) add(24 * NN, 2);                            // x64, d64
   if (mode == 1) add(8 * NN, 13);                    // MMA state
   add(576 * 8);                                      // Ke
   add(64LL * kTri * 8);                              // level-2 templates
@@ -413,8 +427,8 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
   return b + 4096 * nalloc;
 }
 
-int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, const uint8_t* fixState, int nLoad,
-          const int* loadNodes, const double* loadVec) {
+int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, const uint8_t* fixState, int nCases,
+          const int* nPerCase, const int* loadNodes, const double* loadVec) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     P.err = "no HIP device visible: libt3x has no CPU path";
@@ -428,18 +442,25 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     return T3X_ERR_INPUT;
   }
   // ---- boundary conditions (536-557 or explicit), 1294-1336
-  std::vector<long long> fixD, loadD;
+  std::vector<long long> fixD;
+  std::vector<std::vector<long long>> loadD;
   std::vector<uint8_t> fs;
-  std::vector<double> lv;
+  std::vector<std::vector<double>> lv;
   if (nFix < 0) {
+    loadD.resize(1);
+    lv.resize(1);
     for (int jz = 0; jz <= P.nelz; ++jz)
       for (int jy = 0; jy <= P.nely; ++jy) fixD.push_back(jy + (long long)F.NY * (0 + (long long)F.NX * jz));
     fs.assign(fixD.size() * 3, 1);
     const int zTop = (int)std::llround(P.nelz / 6.0 + 1.0);
     for (int jz = 0; jz < zTop; ++jz)
-      for (int jy = 0; jy <= P.nely; ++jy) loadD.push_back(jy + (long long)F.NY * (P.nelx + (long long)F.NX * jz));
-    const double qv = 1.0 / (double)loadD.size();
-    for (size_t i = 0; i < loadD.size(); ++i) { lv.push_back(0.0 * qv); lv.push_back(0.0 * qv); lv.push_back(-1.0 * qv); }
+      for (int jy = 0; jy <= P.nely; ++jy)
+        loadD[0].push_back(jy + (long long)F.NY * (P.nelx + (long long)F.NX * jz));
+    const double qv = 1.0 / (double)loadD[0].size();
+    for (size_t i = 0; i < loadD[0].size(); ++i) {
+      lv[0].push_back(0.0 * qv); lv[0].push_back(0.0 * qv); lv[0].push_back(-1.0 * qv);
+    }
+    P.wObj.assign(1, 1.0);  // 558
   } else {
     const long long uy = P.nely + 1, ux = P.nelx + 1;
     auto adapt = [&](int id) {
@@ -450,23 +471,36 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       fixD.push_back(adapt(fixNodes[i]));
       for (int c = 0; c < 3; ++c) fs.push_back(fixState[3 * i + c]);
     }
-    for (int i = 0; i < nLoad; ++i) {
-      loadD.push_back(adapt(loadNodes[i]));
-      for (int c = 0; c < 3; ++c) lv.push_back(loadVec[3 * i + c]);
-    }
+    loadD.resize(nCases);
+    lv.resize(nCases);
+    long long off = 0;
+    for (int ii = 0; ii < nCases; ++ii)
+      for (int i = 0; i < nPerCase[ii]; ++i, ++off) {
+        loadD[ii].push_back(adapt(loadNodes[off]));
+        for (int c = 0; c < 3; ++c) lv[ii].push_back(loadVec[3 * off + c]);
+      }
+    P.wObj.assign(nCases, 1.0 / nCases);  // 493
   }
+  P.nCases = (int)loadD.size();
   std::vector<uint8_t> hfix(NN, 0);
-  std::vector<double> hF(3 * NN, 0.0);
-  std::vector<uint8_t> loaded(NN, 0);
-  int nL = 0, nF = 0;
-  // a node listed twice keeps its first entry (unique(), 499-518)
-  for (size_t i = 0; i < loadD.size(); ++i) {
-    const long long d = loadD[i];
-    if (d < 0 || d >= NN || !F.hNmask[d] || loaded[d]) continue;
-    loaded[d] = 1;
-    for (int c = 0; c < 3; ++c) hF[c * NN + d] = lv[3 * i + c];
+  std::vector<std::vector<double>> hF(P.nCases);
+  std::vector<uint8_t> loaded(NN, 0);  // loaded in any case: released from the fixings (1301-1305)
+  int nF = 0;
+  for (int ii = 0; ii < P.nCases; ++ii) {
+    // a node listed twice keeps its first entry (unique(), 499-518)
+    std::vector<uint8_t> seen(NN, 0);
+    hF[ii].assign(3 * NN, 0.0);
+    int nL = 0;
+    for (size_t i = 0; i < loadD[ii].size(); ++i) {
+      const long long d = loadD[ii][i];
+      if (d < 0 || d >= NN || !F.hNmask[d] || seen[d]) continue;
+      seen[d] = 1;
+      loaded[d] = 1;
+      ++nL;
+      for (int c = 0; c < 3; ++c) hF[ii][c * NN + d] = lv[ii][3 * i + c];
+    }
+    if (nL == 0) { P.err = "No Loads!"; return T3X_ERR_NO_LOADS; }  // 1309
   }
-  for (long long d = 0; d < NN; ++d) nL += loaded[d];
   std::vector<uint8_t> fixSeen(NN, 0);
   for (size_t i = 0; i < fixD.size(); ++i) {
     const long long d = fixD[i];
@@ -478,7 +512,6 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     hfix[d] = m;
   }
   for (long long d = 0; d < NN; ++d) nF += hfix[d] ? 1 : 0;
-  if (nL == 0) { P.err = "No Loads!"; return T3X_ERR_NO_LOADS; }
   if (nF == 0) { P.err = "No Fixations!"; return T3X_ERR_NO_FIX; }
 
   T3X_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
@@ -495,6 +528,7 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       V.x = P.alloc<double>(3 * V.NN);
       V.Ks = P.alloc<double>((size_t)kTri * V.NN);
     }
+    if (P.standard() && !P.mixed() && l >= 1 && l < P.L - 1) V.t = P.alloc<double>(3 * V.NN);
     if (P.mixed()) {
       if (l < P.L - 1) V.diagf = P.alloc<float>(3 * V.NN);
       if (l >= 1) {
@@ -510,15 +544,22 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   {
     long long tl = 0;
     for (int l = 0; l + 1 < P.L; ++l) tl = std::max(tl, restrict_tmp_len(geo(P.lv[l]), geo(P.lv[l + 1])));
-    if (P.standard())  // also the per-cell products of the coarse smoothers (floats)
-      for (int l = 1; l + 1 < P.L; ++l) tl = std::max(tl, (coarse_ks_tmp_len(geo(P.lv[l])) + 1) / 2);
+    if (P.standard())  // also the per-cell products of the coarse smoothers (floats when mixed)
+      for (int l = 1; l + 1 < P.L; ++l)
+        tl = std::max(tl, P.mixed() ? (coarse_ks_tmp_len(geo(P.lv[l])) + 1) / 2 : coarse_ks_tmp_len(geo(P.lv[l])));
     P.rtmp = P.alloc<double>(tl);
   }
   P.fix = P.alloc<uint8_t>(NN);
   P.upload(P.fix, hfix.data(), NN);
-  P.F = P.alloc<double>(3 * NN);
-  P.upload(P.F, hF.data(), 3 * NN * sizeof(double));
-  for (double** v : {&P.U, &P.r, &P.z, &P.p, &P.q, &P.tmp3}) *v = P.alloc<double>(3 * NN);
+  for (int ii = 0; ii < P.nCases; ++ii) {
+    P.Fc.push_back(P.alloc<double>(3 * NN));
+    P.upload(P.Fc[ii], hF[ii].data(), 3 * NN * sizeof(double));
+    P.Uc.push_back(P.alloc<double>(3 * NN));
+  }
+  P.F = P.Fc[0];
+  P.U = P.Uc[0];
+  if (P.nCases > 1) P.ceTmp = P.alloc<double>(NN);
+  for (double** v : {&P.r, &P.z, &P.p, &P.q, &P.tmp3}) *v = P.alloc<double>(3 * NN);
   for (double** v : {&P.E, &P.x, &P.xPhys, &P.ce, &P.dc, &P.ca, &P.cb, &P.cc, &P.cd}) *v = P.alloc<double>(NN);
   for (double** v : {&P.fb, &P.fx, &P.fr, &P.fz, &P.fp, &P.fq}) *v = P.alloc<double>(NN);
   // ---- Ke and L2 templates
@@ -526,7 +567,11 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   P.dKe = P.alloc<double>(576);
   P.upload(P.dKe, Ke.data(), 576 * sizeof(double));
   if (P.mixed()) P.z32 = P.alloc<float>(3 * NN);
-  if (P.standard()) {
+  if (P.standard() && !P.mixed()) {
+    P.x64 = P.alloc<double>(3 * NN);
+    P.d64 = P.alloc<double>(3 * NN);
+  }
+  if (P.standard() && P.mixed()) {
     std::vector<float> Kf(Ke.begin(), Ke.end());
     P.Ke32 = P.alloc<float>(576);
     P.upload(P.Ke32, Kf.data(), 576 * sizeof(float));
@@ -657,7 +702,7 @@ int assemble(t3x_problem& P) {
   launch_diag_fine(s, g0, P.dKe, P.E, P.fix, P.lv[0].nmask, P.lv[0].diag);
   if (P.mixed())
     for (int l = 0; l < P.L - 1; ++l) launch_convert(s, P.lv[l].diagf, P.lv[l].diag, 3 * P.lv[l].NN);
-  if (P.standard()) {
+  if (P.standard() && P.mixed()) {
     launch_convert(s, P.E32, P.E, P.NN());
     for (int l = 1; l < P.L - 1; ++l) launch_convert(s, P.lv[l].Ksf, P.lv[l].Ks, (long long)kTri * P.lv[l].NN);
   }
@@ -767,6 +812,43 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   P.tend(TG_VCYCLE, ev);
 }
 
+// 'Standard' V-cycle (759-777) in FP64: the same steps as the FP32 form above
+// with the FP64 Galerkin matrices, diagonals and level vectors.
+void vcycle_standard64(t3x_problem& P, const double* r, double* z, double* sc, const int* fl, bool fuse) {
+  hipStream_t s = P.stream;
+  const int L = P.L;
+  const double om = P.P.omega;
+  double* tmp = (double*)P.rtmp;
+  const Geo g0 = geo(P.lv[0]);
+  hipEvent_t ev[2], ec[2];
+  P.tbegin(TG_VCYCLE, ev);
+  launch_jacobi0(s, r, (const double*)P.lv[0].diag, P.x64, om, 3 * P.lv[0].NN, fl);          // 757
+  launch_fine_resid64(s, g0, P.dKe, P.E, P.fix, P.x64, r, P.d64, fl);                         // 761
+  launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.d64, tmp, P.lv[1].nmask, P.lv[1].r, fl);
+  for (int l = 1; l < L - 1; ++l) {
+    Level& V = P.lv[l];
+    launch_jacobi0(s, (const double*)V.r, (const double*)V.diag, V.x, om, 3 * V.NN, fl);
+    launch_coarse_resid64(s, geo(V), V.Ks, V.x, V.r, V.t, tmp, fl);
+    launch_restrict(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const double*)V.t, tmp, P.lv[l + 1].nmask,
+                    P.lv[l + 1].r, fl);
+  }
+  Level& C = P.lv[L - 1];
+  P.tbegin(TG_COARSE, ec);
+  P.cs.solve(s, P.dCMap, C.r, C.x, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);           // 767
+  P.tend(TG_COARSE, ec);
+  for (int l = L - 2; l >= 1; --l) {
+    Level& V = P.lv[l];
+    launch_prolong_add(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const double*)P.lv[l + 1].x, V.x, V.nmask,
+                       fl);                                                                   // 770
+    launch_coarse_smooth64(s, geo(V), V.Ks, V.x, V.r, V.diag, om, V.t, tmp, fl);             // 775
+    std::swap(V.x, V.t);
+  }
+  launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.lv[1].x, P.x64, P.lv[0].nmask, fl);
+  launch_fine_smooth64(s, g0, P.dKe, P.E, P.fix, P.x64, r, P.lv[0].diag, om, z, fuse ? P.ws.partials : nullptr, fl);
+  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_tiled_blocks(g0), P.ws, sc, (int*)fl, nullptr);
+  P.tend(TG_VCYCLE, ev);
+}
+
 // ------------------------------------------------------------- PCG
 // Solving_PCG (600-639) with device-side scalars and convergence flag.
 int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxIT, double* resOut, bool* warned) {
@@ -785,7 +867,8 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
     else vcycle_mixed(P, P.r, P.z32, sc, nullptr, true);
     launch_convert(s, P.p, P.z32, n3);                                                      // 611
   } else {
-    vcycle(P, P.r, P.z, sc, nullptr, true);
+    if (P.standard()) vcycle_standard64(P, P.r, P.z, sc, nullptr, true);
+    else vcycle(P, P.r, P.z, sc, nullptr, true);
     launch_copy(s, P.p, P.z, n3);
   }
   const int ce = std::max(1, P.P.checkEvery);
@@ -796,7 +879,7 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
     P.tbegin(TG_APPLYK, ek);
     launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                 // 614-615
     P.tend(TG_APPLYK, ek);
-    if (P.standard())  // 617-622, with the first fine Jacobi step of 628-629 fused
+    if (mx && P.standard())  // 617-622, with the first fine Jacobi step of 628-629 fused
       launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl, P.lv[0].diagf, P.x32, P.P.omega);
     else
       launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                             // 617-622
@@ -805,7 +888,8 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
       else vcycle_mixed(P, P.r, P.z32, sc, fl, true);
       launch_pcg_pupdate(s, y, P.p, P.z32, n3, sc, fl);                                     // 616, 630
     } else {
-      vcycle(P, P.r, P.z, sc, fl, true);
+      if (P.standard()) vcycle_standard64(P, P.r, P.z, sc, fl, true);                         // 628-629
+      else vcycle(P, P.r, P.z, sc, fl, true);
       launch_pcg_pupdate(s, y, P.p, P.z, n3, sc, fl);
     }
     if (it % ce == 0 || it == maxIT) {
@@ -957,6 +1041,24 @@ int mma_update(t3x_problem& P, int iter, const double* dfdx, double gx, const do
 }
 
 // ------------------------------------------------------------- drivers
+// TopOpti_ComputeUnitCompliance (579-598) over the load cases: ce = sum_j
+// ce_j w_j (597), accumulated in case order.  A single case has weight 1
+// (493 with one case, 558), so its ce is ce_0.
+void compliance_cases(t3x_problem& P) {
+  hipStream_t s = P.stream;
+  const uint8_t* em = P.lv[0].emask;
+  const Geo g0 = geo(P.lv[0]);
+  if (P.nCases == 1) {
+    launch_compliance(s, g0, P.dKe, em, P.Uc[0], P.ce);
+    return;
+  }
+  for (int j = 0; j < P.nCases; ++j) {
+    launch_compliance(s, g0, P.dKe, em, P.Uc[j], P.ceTmp);
+    launch_elementwise(s, j == 0 ? OP_MULC : OP_AXPYC, P.NN(), em, P.ce, P.ceTmp, nullptr, P.wObj[j], 0, 0);
+  }
+}
+
+// ------------------------------------------------------------- drivers
 // TOP3D_XL_TO (40-223) / TOP3D_XL_PIO (225-440) split into begin / one
 // design iteration / finish so callers (bench, serving engine) can drive
 // the loop one iteration at a time.
@@ -974,16 +1076,18 @@ int opt_begin(t3x_problem& P, int mode, double V0, int nLoop, double rMin, doubl
   launch_elementwise(s, OP_SET, NN, em, P.xPhys, nullptr, nullptr, V0, 0, 0);  // 83
   launch_elementwise(s, OP_SET, NN, em, P.xTilde(), nullptr, nullptr, V0, 0, 0);  // 279
   launch_elementwise(s, OP_SET, NN, em, P.volMax(), nullptr, nullptr, V0, 0, 0);  // 278
-  launch_fill(s, P.U, 0.0, 3 * NN);                                               // 95
+  for (double* u : P.Uc) launch_fill(s, u, 0.0, 3 * NN);                          // 95
   if (mode == 1) mma_alloc(P);
   // solid reference solve (94-106 / 294-306)
   launch_elementwise(s, OP_SET, NN, em, P.E, nullptr, nullptr, P.P.E0, 0, 0);
   int rc = assemble(P);
   if (rc) return rc;
-  bool warned = false;
-  pcg_elastic(P, P.F, P.U, P.P.tol, P.P.maxIT, nullptr, &warned);
-  if (warned) P.warnings++;
-  launch_compliance(s, geo(P.lv[0]), P.dKe, em, P.U, P.ce);
+  for (int j = 0; j < P.nCases; ++j) {  // 101 / 301: one solve per F_ column
+    bool warned = false;
+    pcg_elastic(P, P.Fc[j], P.Uc[j], P.P.tol, P.P.maxIT, nullptr, &warned);
+    if (warned) P.warnings++;
+  }
+  compliance_cases(P);
   S.cSolid = masked_sum(P, P.E, P.ce, 1);  // 104
   S.active = true;
   return T3X_OK;
@@ -1012,12 +1116,20 @@ int opt_step(t3x_problem& P, t3x_iter* it) {
   int rc = assemble(P);  // 118
   if (rc) return -rc;
   const double tb = wall();
-  bool warned = false;
+  int its = 0;
   double resFinal = 0.0;
-  const int its = pcg_elastic(P, P.F, P.U, P.P.tol, P.P.maxIT, &resFinal, &warned);  // 121
-  if (warned) P.warnings++;
+  for (int j = 0; j < P.nCases; ++j) {  // 121 / 321: one warm-started solve per F_ column
+    bool warned = false;
+    double res = 0.0;
+    const int itj = pcg_elastic(P, P.Fc[j], P.Uc[j], P.P.tol, P.P.maxIT, &res, &warned);
+    if (warned) P.warnings++;
+    S.lssIts.push_back(itj);
+    S.resAll.push_back(res);
+    its += itj;
+    resFinal = std::max(resFinal, res);
+  }
   const double tcg = wall();
-  launch_compliance(s, geo(P.lv[0]), P.dKe, em, P.U, P.ce);                  // 124
+  compliance_cases(P);                                                       // 124
   launch_elementwise(s, OP_DIVC, NN, em, tmp, P.ce, nullptr, cSolid, 0, 0);  // 125 ceNorm
   const double cObj = masked_sum(P, P.E, tmp, 1);                            // 126
   const double cDesign = cObj * cSolid;                                      // 127
@@ -1117,9 +1229,11 @@ int opt_result(t3x_problem& P, t3x_result* O) {
     if (O->volHist) O->volHist[k] = r.vol;
     if (O->consHist) O->consHist[k] = r.cons;
     if (O->sharpHist) O->sharpHist[k] = r.sharp;
-    if (O->lssIts) O->lssIts[k] = r.pcgIts;
-    if (O->resHist) O->resHist[k] = r.res;
     if (O->tHist) std::memcpy(O->tHist + (size_t)k * 6, r.t, 6 * sizeof(double));
+  }
+  for (size_t k = 0; k < S.lssIts.size(); ++k) {  // one entry per solve, case fastest
+    if (O->lssIts) O->lssIts[k] = S.lssIts[k];
+    if (O->resHist) O->resHist[k] = S.resAll[k];
   }
   if (O->xPhys) {
     launch_gather_compact(s, P.dCellMap, P.nCells, 1, NN, P.xPhys, P.tmp3);
@@ -1211,9 +1325,9 @@ void t3x_default_params(t3x_params* p) {
   p->vcycle = 0;  // 'Adapted' (34)
 }
 
-int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
-               const uint8_t* fixState, int nLoad, const int* loadNodes, const double* loadVec,
-               const t3x_params* params, t3x_problem** out) {
+int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                     const uint8_t* fixState, int nCases, const int* nPerCase, const int* loadNodes,
+                     const double* loadVec, const t3x_params* params, t3x_problem** out) {
   if (!out) return T3X_ERR_INPUT;
   *out = nullptr;
   std::unique_ptr<t3x_problem> P(new t3x_problem());
@@ -1224,15 +1338,19 @@ int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, co
     *out = P.release();
     return T3X_ERR_INPUT;
   }
-  if ((P->P.precision != 0 && P->P.precision != 1) || (P->P.vcycle != 0 && P->P.vcycle != 1) ||
-      (P->P.vcycle == 1 && P->P.precision != 1)) {
-    P->err = "params: precision must be 0 (FP64) or 1 (mixed); vcycle 1 ('Standard') needs precision 1";
+  if ((P->P.precision != 0 && P->P.precision != 1) || (P->P.vcycle != 0 && P->P.vcycle != 1)) {
+    P->err = "params: precision must be 0 (FP64) or 1 (mixed); vcycle 0 ('Adapted') or 1 ('Standard')";
     *out = P.release();
     return T3X_ERR_INPUT;
   }
   int rc;
   try {
-    rc = setup(*P, voxels, nFix, fixNodes, fixState, nLoad, loadNodes, loadVec);
+    if (nFix >= 0 && (nCases < 1 || !nPerCase)) {
+      P->err = "No Loads!";
+      rc = T3X_ERR_NO_LOADS;
+    } else {
+      rc = setup(*P, voxels, nFix, fixNodes, fixState, nCases, nPerCase, loadNodes, loadVec);
+    }
   } catch (const HipError& e) {
     P->err = e.msg;
     rc = T3X_ERR_HIP;
@@ -1240,6 +1358,15 @@ int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, co
   *out = P.release();
   return rc;
 }
+
+int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+               const uint8_t* fixState, int nLoad, const int* loadNodes, const double* loadVec,
+               const t3x_params* params, t3x_problem** out) {
+  return t3x_create_multi(voxels, nely, nelx, nelz, nFix, fixNodes, fixState, 1, &nLoad, loadNodes, loadVec, params,
+                          out);
+}
+
+int t3x_num_load_cases(const t3x_problem* h) { return h ? h->nCases : 0; }
 
 void t3x_destroy(t3x_problem* h) { delete h; }
 
@@ -1414,7 +1541,8 @@ int t3x_vcycle(t3x_problem* h, const double* r, double* x) {
       else vcycle_mixed(*h, h->r, h->z32, h->sc, nullptr, false);
       launch_convert(h->stream, h->z, h->z32, 3 * h->NN());
     } else {
-      vcycle(*h, h->r, h->z, h->sc, nullptr, false);
+      if (h->standard()) vcycle_standard64(*h, h->r, h->z, h->sc, nullptr, false);
+      else vcycle(*h, h->r, h->z, h->sc, nullptr, false);
     }
     download_nodal(*h, h->z, x);
     return (int)T3X_OK;

@@ -55,7 +55,8 @@ struct Level {
   double* Ks = nullptr;      // kTri*NN, Galerkin cell matrices (levels >= 2), entry-major
   float* rf = nullptr;       // mixed-precision V-cycle vectors
   float* xf = nullptr;
-  float* tf = nullptr;       // 'Standard' cycle: defect / smoother output (levels 1..L-2)
+  double* t = nullptr;       // FP64 'Standard' cycle: defect / smoother output (levels 1..L-2)
+  float* tf = nullptr;       // FP32 'Standard' cycle: defect / smoother output (levels 1..L-2)
   float* diagf = nullptr;
   float* Ksf = nullptr;      // FP32 copy of Ks (levels 1..L-2, 'Standard' cycle)
 };

@@ -43,6 +43,13 @@ void launch_fine_smooth32_mfma(hipStream_t s, const Geo& g, const float* Ke, con
 int applyk_tiled_blocks(const Geo& g);
 void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                           const double* u, double* y, double* partials, const int* fl);
+// FP64 'Standard' smoother products: d = r - Kx; out = x + omega (r - Kx) / diag
+// (partials: out.r per work-group, nullptr: none)
+void launch_fine_resid64(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                         const double* x, const double* r, double* d, const int* fl);
+void launch_fine_smooth64(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+                          const double* x, const double* r, const double* diag, double omega, double* out,
+                          double* partials, const int* fl);
 void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                            const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
@@ -80,21 +87,30 @@ void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span
 void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const float* xc,
                            const double* rf, const float* diag, const uint8_t* nmaskF, float* out,
                            const RedWs* ws, double* sc, const int* fl);
-// 'Standard' V-cycle pieces (FP32): x += P xc; x = omega r / d
+// 'Standard' V-cycle pieces: x += P xc; x = omega r / d
 void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* xc, float* x,
                         const uint8_t* nmaskF, const int* fl);
+void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* xc, double* x,
+                        const uint8_t* nmaskF, const int* fl);
+void launch_jacobi0(hipStream_t s, const double* r, const double* d, double* x, double omega, long long n,
+                    const int* fl);
 void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
                     const int* fl);
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
                     const int* fl);
-// FP32 K*u smoothers of coarse levels with the stored Galerkin matrices (FP32
-// copy; t3x_applyk.hip).  resid: d = r - Kx; smooth: out = x + omega (r - Kx) / diag.
-// tmp: coarse_ks_tmp_len(g) floats of scratch (per-cell products).
+// K*u smoothers of coarse levels with the stored Galerkin matrices (FP32 copy
+// or the FP64 originals; t3x_applyk.hip).  resid: d = r - Kx; smooth:
+// out = x + omega (r - Kx) / diag.  tmp: coarse_ks_tmp_len(g) elements of
+// scratch (per-cell products).
 long long coarse_ks_tmp_len(const Geo& g);
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
                            float* tmp, const int* fl);
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
                             const float* diag, double omega, float* out, float* tmp, const int* fl);
+void launch_coarse_resid64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
+                           double* d, double* tmp, const int* fl);
+void launch_coarse_smooth64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
+                            const double* diag, double omega, double* out, double* tmp, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const float* r, double* v, const int* fl);
 void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long NN3,
@@ -163,7 +179,7 @@ void launch_set_where(hipStream_t s, long long n, const uint8_t* mask, double* a
 void launch_elementwise(hipStream_t s, int op, long long n, const uint8_t* emask, double* a, const double* b,
                         const double* c, double p0, double p1, double p2);
 enum { OP_DC = 0, OP_MUL, OP_DIVMAX, OP_HEAVI, OP_DHEAVI, OP_PNORMGRAD, OP_COPY, OP_SET, OP_DIVC,
-       OP_MMA_BOUNDS };
+       OP_MMA_BOUNDS, OP_MULC, OP_AXPYC };
 enum MmaSlot {
   MM_LAM = 0, MM_MU, MM_EPSI, MM_ERR, MM_LOOP, MM_DONE, MM_B, MM_Y, MM_Z, MM_GRAD, MM_HESS, MM_S1, MM_S2,
   MM_TOL, MM_C, MM_A, MM_NEWTON, MM_GX, MM_COUNT

@@ -1283,6 +1283,8 @@ __global__ void k_elementwise(int op, long long n, const uint8_t* __restrict__ e
     case OP_COPY: a[i] = b[i]; break;
     case OP_SET: a[i] = p0; break;
     case OP_DIVC: a[i] = b[i] / p0; break;
+    case OP_MULC: a[i] = b[i] * p0; break;
+    case OP_AXPYC: a[i] = __dadd_rn(a[i], __dmul_rn(b[i], p0)); break;  // a + b*p0, rounded twice (597)
     case OP_MMA_BOUNDS: {  // a = xmin = max(0, x - move), c-> xmax written by the caller's second call
       a[i] = p1 > 0.0 ? fmin(1.0, b[i] + p0) : fmax(0.0, b[i] - p0);  // 364-365
     } break;

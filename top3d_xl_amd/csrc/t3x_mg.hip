@@ -267,22 +267,36 @@ void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span
   prolong_impl(s, gf, gc, span, omega, xc, rf, diag, nmaskF, out, ws, sc, fl);
 }
 
-void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* xc, float* x,
-                        const uint8_t* nmaskF, const int* fl) {
+template <typename T>
+static void prolong_add_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const T* xc, T* x,
+                             const uint8_t* nmaskF, const int* fl) {
   int zc = 0;
   const dim3 g = prolong_grid(gf, &zc);
   if (span == 4)
-    hipLaunchKernelGGL((k_prolong<4, float, float, float, float, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
-                       (const float*)nullptr, (const float*)nullptr, nmaskF, x, (double*)nullptr, fl);
+    hipLaunchKernelGGL((k_prolong<4, T, T, T, T, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
+                       (const T*)nullptr, (const T*)nullptr, nmaskF, x, (double*)nullptr, fl);
   else
-    hipLaunchKernelGGL((k_prolong<2, float, float, float, float, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
-                       (const float*)nullptr, (const float*)nullptr, nmaskF, x, (double*)nullptr, fl);
+    hipLaunchKernelGGL((k_prolong<2, T, T, T, T, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
+                       (const T*)nullptr, (const T*)nullptr, nmaskF, x, (double*)nullptr, fl);
+}
+void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* xc, float* x,
+                        const uint8_t* nmaskF, const int* fl) {
+  prolong_add_impl(s, gf, gc, span, xc, x, nmaskF, fl);
+}
+void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* xc, double* x,
+                        const uint8_t* nmaskF, const int* fl) {
+  prolong_add_impl(s, gf, gc, span, xc, x, nmaskF, fl);
 }
 
 void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
                     const int* fl) {
   hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
                      n, fl);
+}
+void launch_jacobi0(hipStream_t s, const double* r, const double* d, double* x, double omega, long long n,
+                    const int* fl) {
+  hipLaunchKernelGGL((k_jacobi0<double, double, double>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x,
+                     omega, n, fl);
 }
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
                     const int* fl) {

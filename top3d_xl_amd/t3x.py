@@ -79,6 +79,9 @@ def load_library():
     L.t3x_default_params.argtypes = [C.POINTER(Params)]
     L.t3x_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                              C.c_void_p, C.c_void_p, C.POINTER(Params), C.POINTER(vp)]
+    L.t3x_create_multi.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Params), C.POINTER(vp)]
+    L.t3x_num_load_cases.argtypes = [vp]
     L.t3x_destroy.argtypes = [vp]
     L.t3x_last_error.restype = C.c_char_p
     L.t3x_last_error.argtypes = [vp]
@@ -116,7 +119,7 @@ def load_library():
     return L
 
 
-EXPORTED = ["t3x_default_params", "t3x_create", "t3x_destroy", "t3x_last_error", "t3x_info", "t3x_run",
+EXPORTED = ["t3x_default_params", "t3x_create", "t3x_create_multi", "t3x_num_load_cases", "t3x_destroy", "t3x_last_error", "t3x_info", "t3x_run",
             "t3x_opt_begin", "t3x_opt_step", "t3x_opt_result", "t3x_kernel_time", "t3x_kernel_time_reset",
             "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance",
             "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy", "t3x_set_passive",
@@ -139,7 +142,10 @@ class T3XError(RuntimeError):
 class Problem:
     """One voxel model on the GPU (state after CreateVoxelFEAmodel + FEA set-up)."""
 
-    def __init__(self, voxels, fix=None, load=None, **params):
+    def __init__(self, voxels, fix=None, load=None, extra_loads=(), **params):
+        """fix = (nodes, state n x 3), load = (nodes, vec n x 3) on the unpadded
+        node grid (0-based); extra_loads: further load cases (nodes, vec)
+        (loadingCond_{2..}, 483-493).  fix None: the built-in cuboid."""
         L = load_library()
         vox = np.asfortranarray(np.asarray(voxels).astype(np.uint8))
         if vox.ndim != 3:
@@ -155,11 +161,15 @@ class Problem:
         else:
             fn = np.ascontiguousarray(fix[0], dtype=np.int32)
             fs = np.ascontiguousarray(fix[1], dtype=np.uint8).reshape(-1)
-            ln = np.ascontiguousarray(load[0], dtype=np.int32)
-            lv = np.ascontiguousarray(load[1], dtype=np.float64).reshape(-1)
-            self._keep += [fn, fs, ln, lv]
-            rc = L.t3x_create(flat.ctypes.data, nely, nelx, nelz, len(fn), fn.ctypes.data, fs.ctypes.data, len(ln),
-                              ln.ctypes.data, lv.ctypes.data, C.byref(P), C.byref(h))
+            cases = [load] + list(extra_loads)
+            npc = np.array([len(np.asarray(c[0]).reshape(-1)) for c in cases], dtype=np.int32)
+            ln = np.ascontiguousarray(np.concatenate([np.asarray(c[0]).reshape(-1) for c in cases]), dtype=np.int32)
+            lv = np.ascontiguousarray(np.concatenate([np.asarray(c[1], dtype=np.float64).reshape(-1)
+                                                      for c in cases]), dtype=np.float64)
+            self._keep += [fn, fs, npc, ln, lv]
+            rc = L.t3x_create_multi(flat.ctypes.data, nely, nelx, nelz, len(fn), fn.ctypes.data, fs.ctypes.data,
+                                    len(cases), npc.ctypes.data, ln.ctypes.data, lv.ctypes.data, C.byref(P),
+                                    C.byref(h))
         self.L = L
         self.h = h
         if rc != T3X_OK:
@@ -168,6 +178,7 @@ class Problem:
                 L.t3x_destroy(h)
             self.h = None
             raise T3XError(f"t3x_create failed ({_STATUS.get(rc, rc)}): {msg}")
+        self.num_cases = L.t3x_num_load_cases(h)
         info = np.zeros(1 + 8 * 16, dtype=np.int32)
         L.t3x_info(h, info.ctypes.data)
         self.num_levels = int(info[0])
@@ -287,18 +298,19 @@ class Problem:
         nE = self.nE
         n = max(nLoop, 1)
         xPhys = np.zeros(nE)
+        nc = self.num_cases
         cH, vH, conH, sH = (np.zeros(n) for _ in range(4))
-        its = np.zeros(n, dtype=np.int32)
+        its = np.zeros(n * nc, dtype=np.int32)
         tH = np.zeros(n * 6)
         snap = np.zeros(max(nSnap, 1) * nE)
-        resH = np.zeros(n)
+        resH = np.zeros(n * nc)
         R = Result(xPhys.ctypes.data, cH.ctypes.data, vH.ctypes.data, conH.ctypes.data, sH.ctypes.data,
                    its.ctypes.data, tH.ctypes.data, snap.ctypes.data, int(nSnap), resH.ctypes.data, 0.0, 0, 0)
         m = {"GLOBAL": 0, "LOCAL": 1}[mode] if isinstance(mode, str) else int(mode)
         self._check(self.L.t3x_run(self.h, m, float(V0), int(nLoop), float(rMin), float(rHat), C.byref(R)))
         k = R.loops
         return dict(xPhys=xPhys, cHist=cH[:k], volHist=vH[:k], consHist=conH[:k], sharpHist=sH[:k],
-                    lssIts=its[:k], resHist=resH[:k], tHist=tH[:6 * k].reshape(k, 6),
+                    lssIts=its[:k * nc], resHist=resH[:k * nc], tHist=tH[:6 * k].reshape(k, 6),
                     xSnap=snap[:min(nSnap, k) * nE].reshape(-1, nE),
                     cSolid=R.cSolid, loops=k, warnings=R.warnings)
 
@@ -306,9 +318,13 @@ class Problem:
 def TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin, **params):
     """Drop-in for TOP3D_XL.m:4-19.  inputModel is a logical (nely,nelx,nelz)
     array (the built-in cuboid BCs apply, 529-560) or a (voxels, fix, load)
-    tuple for explicit boundary conditions."""
+    or (voxels, fix, load, extra_loads) tuple for explicit boundary
+    conditions and further load cases."""
+    extra = ()
     if isinstance(inputModel, tuple):
-        vox, fix, load = inputModel
+        vox, fix, load = inputModel[:3]
+        if len(inputModel) > 3:
+            extra = inputModel[3]
     else:
         vox, fix, load = inputModel, None, None
     if consType == "GLOBAL":
@@ -321,7 +337,7 @@ def TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin, **params):
         mode = 1
     else:
         raise T3XError("Wrong Input!")
-    P = Problem(vox, fix, load, **params)
+    P = Problem(vox, fix, load, extra_loads=extra, **params)
     try:
         return P.run(mode, V0, nLoop, rMin, rHat)
     finally:

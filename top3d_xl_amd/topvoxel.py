@@ -111,8 +111,23 @@ def read_topvoxel(path):
                 load_nodes=load_nodes, load_vec=np.asarray(load_vec, dtype=np.float64), extra_loads=extra)
 
 
-def write_topvoxel(path, voxels, fix_nodes, fix_state, load_nodes, load_vec, passive=None, density=None):
-    """Write a version-1.0 .TopVoxel file (0-based ids in, 1-based ids out)."""
+def load_cases(model):
+    """The load cases of a parsed model in loadingCond_ order: case 1 from
+    "Loads:", then each additional case at its index idx (490).  The indices
+    must fill 2..n+1: a gap leaves an empty case, which the reference rejects
+    with 'No Loads!' (1309)."""
+    cases = [(model["load_nodes"], model["load_vec"])]
+    extra = sorted(model.get("extra_loads", []), key=lambda t: t[0])
+    if [t[0] for t in extra] != list(range(2, 2 + len(extra))):
+        raise FormatError("No Loads!")
+    cases += [(nodes, vec) for _, nodes, vec in extra]
+    return cases
+
+
+def write_topvoxel(path, voxels, fix_nodes, fix_state, load_nodes, load_vec, passive=None, density=None,
+                   extra_loads=()):
+    """Write a version-1.0 .TopVoxel file (0-based ids in, 1-based ids out);
+    extra_loads: further load cases (nodes, vec), written as cases 2, 3, ..."""
     vox = np.asarray(voxels, dtype=bool)
     nely, nelx, nelz = vox.shape
     solid = np.flatnonzero(vox.reshape(-1, order="F"))
@@ -140,4 +155,9 @@ def write_topvoxel(path, voxels, fix_nodes, fix_state, load_nodes, load_vec, pas
         f.write(f"Loads: {len(load_nodes)}\n")
         for n, v in zip(load_nodes, load_vec):
             f.write(f"{int(n) + 1} {v[0]:.16e} {v[1]:.16e} {v[2]:.16e}\n")
-        f.write("Additional Loads: 0\n")
+        f.write(f"Additional Loads: {len(extra_loads)}\n")
+        for k, (nodes, vec) in enumerate(extra_loads):
+            vec = np.asarray(vec, dtype=np.float64).reshape(-1, 3)
+            f.write(f"Loads: {len(nodes)} {k + 2}\n")
+            for n, v in zip(nodes, vec):
+                f.write(f"{int(n) + 1} {v[0]:.16e} {v[1]:.16e} {v[2]:.16e}\n")
