@@ -102,6 +102,20 @@ const char* t3x_last_error(const t3x_problem* h);
  * (274-276, 366, 381).  Call before t3x_run / t3x_opt_begin; void voxels are
  * ignored; replaces any earlier set. */
 int t3x_set_passive(t3x_problem* h, int n, const int* voxelIds);
+/* TopOpti_SetPassiveElements (1511-1584, called at 79 / 272 with
+ * specifyPassiveRegions_ = [#LayersBoundary, #LayersLoads, #LayersFixations],
+ * 37): adds to the passive set numLayerBoundary layers of cells from the
+ * domain boundary inwards, and numLayerLoads / numLayerFixation layers
+ * around the loaded (any case) / fixed nodes.  Layer 1 is the boundary cells
+ * (a node shared by fewer than 8 solid cells, 1714-1728), the ones touching
+ * those nodes; each further layer adds the 3x3x3 neighbourhood within the
+ * solid cells (Common_IncludeAdjacentElements, 1587-1615).  Call after
+ * t3x_set_passive (which replaces the set) and before t3x_run. */
+int t3x_set_passive_layers(t3x_problem* h, int numLayerBoundary, int numLayerLoads, int numLayerFixation);
+/* The passive set (passiveElements_): *count voxel ids, ascending, 0-based on
+ * the unpadded grid; at most cap are written to voxelIds (which may be NULL
+ * to query the count). */
+int t3x_get_passive(const t3x_problem* h, int cap, int* voxelIds, int* count);
 /* info[0]=numLevels; per level l at 1+8l: resX,resY,resZ,span,numElements,numNodes,0,0 */
 int t3x_info(const t3x_problem* h, int* info);
 

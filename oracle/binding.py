@@ -66,6 +66,7 @@ def _load():
     L.orc_num_threads.restype = C.c_int
     L.orc_set_order_variant.argtypes = [vp, C.c_int]
     L.orc_set_passive.argtypes = [vp, C.c_int, _ip]
+    L.orc_set_passive_layers.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.orc_set_vcycle.argtypes = [vp, C.c_int]
     L.orc_set_omega.argtypes = [vp, C.c_double]
     L.orc_apply_K_level.argtypes = [vp, C.c_int, _dp, _dp]
@@ -132,7 +133,8 @@ class Oracle:
     _types = {"transferMatCoeffi": np.float64, "diagK": np.float64, "Ks": np.float64, "RI": np.float64,
               "Ke": np.float64, "F": np.float64, "objWeight": np.float64, "KesFree": np.float64,
               "KesFixed": np.float64,
-              "filterKernel0": np.float64, "filterDiag0": np.float64, "fixState": np.uint8}
+              "filterKernel0": np.float64, "filterDiag0": np.float64, "fixState": np.uint8,
+              "isPassive": np.uint8}
 
     def get(self, name: str, level: int = 0) -> np.ndarray:
         n = self.L.orc_get(self.h, name.encode(), level, None, 0)
@@ -261,6 +263,10 @@ class Oracle:
         """passiveElements_: 0-based voxel ids of the unpadded grid."""
         ids = np.ascontiguousarray(voxel_ids, dtype=np.int32)
         self.L.orc_set_passive(self.h, len(ids), ids)
+
+    def set_passive_layers(self, n_boundary, n_loads, n_fixations):
+        """specifyPassiveRegions_ (37) through TopOpti_SetPassiveElements."""
+        self.L.orc_set_passive_layers(self.h, int(n_boundary), int(n_loads), int(n_fixations))
 
     @staticmethod
     def num_threads():

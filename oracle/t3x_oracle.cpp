@@ -1675,6 +1675,8 @@ long long orc_get(void* h, const char* name, int level, void* out, long long cap
   }
   if (n == "objWeight") return cp(M->objWeight.data(), M->objWeight.size(), sizeof(double));
   if (n == "fixNode") return cp(M->fixNode.data(), M->fixNode.size(), sizeof(int));
+  if (n == "isPassive") return cp(M->isPassive.data(), M->isPassive.size(), 1);
+  if (n == "elementsOnBoundary") return cp(L.elementsOnBoundary.data(), L.elementsOnBoundary.size(), sizeof(int));
   if (n == "fixState") return cp(M->fixState.data(), M->fixState.size(), 1);
   if (n == "mapUniqueKes") return cp(M->mapUniqueKes.data(), M->mapUniqueKes.size(), sizeof(int));
   if (n == "KesFree") return cp(M->KesFree.data(), M->KesFree.size(), sizeof(double));
@@ -1798,6 +1800,65 @@ void orc_set_passive(void* h, int n, const int* voxelIds) {
     const int dense = iy + L.resY * (ix + L.resX * iz);
     const int c = L.eleMapForward[dense];
     if (c > 0) M->isPassive[c - 1] = 1;
+  }
+}
+
+// TopOpti_SetPassiveElements, 1511-1584, on the compact lists: layer 1 from
+// elementsOnBoundary (1526) or from the boundary elements around the loaded
+// / fixed nodes (nodeStruct_, 1536-1546; the nodes are taken as the loaded
+// and fixed compact nodes themselves), further layers by
+// Common_IncludeAdjacentElements (1587-1615): the 3x3x3 neighbourhood of
+// each element on the dense grid, kept where it is solid.
+void orc_set_passive_layers(void* h, int nB, int nLd, int nFx) {
+  Model* M = (Model*)h;
+  const Level& L = M->L[0];
+  if (M->isPassive.size() != (size_t)L.numElements) M->isPassive.assign(L.numElements, 0);
+  auto adjacent = [&](const std::vector<int>& in) {  // 1587-1615
+    std::vector<int> out;
+    for (int e : in) {
+      const int d = L.eleMapBack[e];
+      const int iy = d % L.resY, ix = (d / L.resY) % L.resX, iz = d / (L.resY * L.resX);
+      for (int dz = -1; dz <= 1; ++dz)
+        for (int dx = -1; dx <= 1; ++dx)
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int y = iy + dy, x = ix + dx, z = iz + dz;
+            if (y < 0 || x < 0 || z < 0 || y >= L.resY || x >= L.resX || z >= L.resZ) continue;
+            const int c = L.eleMapForward[y + L.resY * (x + L.resX * z)];
+            if (c > 0) out.push_back(c - 1);
+          }
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  };
+  auto grow = [&](std::vector<int> set, int n) {
+    for (int k = 2; k <= n; ++k) set = adjacent(set);
+    return set;
+  };
+  auto mark = [&](const std::vector<int>& set) { for (int e : set) M->isPassive[e] = 1; };
+  if (nB > 0) mark(grow(L.elementsOnBoundary, nB));
+  if (nLd > 0 || nFx > 0) {
+    std::vector<char> isBE(L.numElements, 0);
+    for (int e : L.elementsOnBoundary) isBE[e] = 1;
+    auto near = [&](const std::vector<char>& nodeFlag) {
+      std::vector<int> out;
+      for (int e = 0; e < L.numElements; ++e) {
+        if (!isBE[e]) continue;
+        for (int j = 0; j < 8; ++j)
+          if (nodeFlag[L.eNodMat[(size_t)e * 8 + j]]) { out.push_back(e); break; }
+      }
+      return out;
+    };
+    if (nLd > 0) {
+      std::vector<char> f(L.numNodes, 0);
+      for (const auto& c : M->loadNode) for (int n : c) f[n] = 1;
+      mark(grow(near(f), nLd));
+    }
+    if (nFx > 0) {
+      std::vector<char> f(L.numNodes, 0);
+      for (int n : M->fixNode) f[n] = 1;
+      mark(grow(near(f), nFx));
+    }
   }
 }
 

@@ -94,3 +94,92 @@ def test_gpu_passive_matches_oracle(mode):
     # densities by O(0.1)
     np.testing.assert_allclose(rg["cHist"], ro["cHist"], rtol=1e-4)
     assert np.abs(rg["xSnap"] - ro["xSnap"]).max() < 1e-4
+
+
+# ------------------------------------------- passive layers (1511-1615)
+def layers_numpy(vox, fixed_nodes, loaded_nodes, nB, nL, nF):
+    """Brute force of TopOpti_SetPassiveElements on the unpadded voxel grid:
+    boundary nodes have fewer than 8 solid cells around them; layer 1 is the
+    solid cells touching such a node (or, near loads / fixations, those of
+    them touching a loaded / fixed node); each further layer is a 3x3x3
+    binary dilation restricted to the solid cells."""
+    from scipy import ndimage
+    vox = np.asarray(vox, bool)
+    ny, nx, nz = vox.shape
+    cnt = np.zeros((ny + 1, nx + 1, nz + 1), int)
+    for dy in (0, 1):
+        for dx in (0, 1):
+            for dz in (0, 1):
+                cnt[dy:dy + ny, dx:dx + nx, dz:dz + nz] += vox
+    bnode = (cnt > 0) & (cnt < 8)
+
+    def touching(nodes):
+        t = np.zeros_like(vox)
+        for dy in (0, 1):
+            for dx in (0, 1):
+                for dz in (0, 1):
+                    t |= nodes[dy:dy + ny, dx:dx + nx, dz:dz + nz]
+        return t & vox
+
+    bcell = touching(bnode)
+
+    def grow(s, n):
+        for _ in range(n - 1):
+            s = ndimage.binary_dilation(s, np.ones((3, 3, 3), bool)) & vox
+        return s
+
+    out = np.zeros_like(vox)
+    if nB > 0:
+        out |= grow(bcell, nB)
+    for n, ids in ((nL, loaded_nodes), (nF, fixed_nodes)):
+        if n > 0:
+            m = np.zeros((ny + 1) * (nx + 1) * (nz + 1), bool)
+            m[np.asarray(ids)] = True
+            out |= grow(bcell & touching(m.reshape((ny + 1, nx + 1, nz + 1), order="F")), n)
+    return np.flatnonzero(out.reshape(-1, order="F"))
+
+
+def layer_model():
+    vox = np.ones((8, 14, 8), bool)
+    vox[3:5, 5:9, 3:5] = False  # an interior cavity: its faces are boundary too
+    ny, nx, nz = vox.shape
+    uy, ux = ny + 1, nx + 1
+    fix = np.array([jy + uy * (0 + ux * jz) for jz in range(nz + 1) for jy in range(ny + 1)])
+    load = np.array([jy + uy * (nx + ux * jz) for jz in range(2) for jy in range(ny + 1)])
+    return vox, (fix, np.ones((len(fix), 3), np.uint8)), (load, np.tile([0, 0, -1.0], (len(load), 1)) / len(load))
+
+
+LAYERS = [(1, 0, 0), (2, 0, 0), (0, 1, 0), (0, 2, 3), (1, 3, 2)]
+
+
+@pytest.mark.usefixtures("oracle_lib")
+@pytest.mark.parametrize("layers", LAYERS)
+def test_oracle_passive_layers_match_dilation(layers):
+    vox, fix, load = layer_model()
+    o = Oracle(vox, fix, load)
+    o.set_passive_layers(*layers)
+    got = np.flatnonzero(o.get("isPassive"))  # compact order == voxel order (no padding here)
+    want = layers_numpy(vox, fix[0], load[0], *layers)
+    ids = np.flatnonzero(vox.reshape(-1, order="F"))  # compact -> voxel id
+    np.testing.assert_array_equal(ids[got], want)
+    assert len(want) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.usefixtures("oracle_lib")
+@pytest.mark.parametrize("layers", [(1, 0, 0), (0, 2, 3)])
+def test_gpu_passive_layers_match_oracle(layers):
+    import top3d_xl_amd as t3x
+    vox, fix, load = layer_model()
+    print(f"\n[gpu-passive] layers {layers}", flush=True)
+    g = t3x.Problem(vox, fix, load)
+    g.set_passive_layers(*layers)
+    np.testing.assert_array_equal(g.passive(), layers_numpy(vox, fix[0], load[0], *layers))
+    rg = g.run("GLOBAL", 0.3, 4, math.sqrt(3), nSnap=4)
+    g.close()
+    o = Oracle(vox, fix, load)
+    o.set_passive_layers(*layers)
+    ro = o.run("GLOBAL", 0.3, 4, math.sqrt(3), nSnap=4)
+    assert rg["loops"] == ro["loops"]
+    np.testing.assert_allclose(rg["cHist"], ro["cHist"], rtol=1e-4)
+    assert np.abs(rg["xSnap"] - ro["xSnap"]).max() < 1e-4

@@ -208,6 +208,9 @@ struct t3x_problem {
   int nNodes = 0, nCells = 0;
   int *dNodeMap = nullptr, *dCellMap = nullptr;
   std::vector<int> hNodeMap, hCellMap;
+  // host copies for the passive-layer generator: nodes left in the fixings,
+  // nodes loaded in any case, the passive cells (dense node/cell indexing)
+  std::vector<uint8_t> hFixNode, hLoaded, hPassive;
   // reductions / scalars
   RedWs ws{nullptr, nullptr};
   double* sc = nullptr;   // [0..15] elastic PCG, [16..31] filter PCG, [32..] misc
@@ -502,17 +505,20 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     if (nL == 0) { P.err = "No Loads!"; return T3X_ERR_NO_LOADS; }  // 1309
   }
   std::vector<uint8_t> fixSeen(NN, 0);
+  P.hFixNode.assign(NN, 0);
   for (size_t i = 0; i < fixD.size(); ++i) {
     const long long d = fixD[i];
     if (d < 0 || d >= NN || !F.hNmask[d] || fixSeen[d]) continue;
     fixSeen[d] = 1;
     if (loaded[d]) continue;  // loaded nodes are released (1303)
+    P.hFixNode[d] = 1;
     uint8_t m = 0;
     for (int c = 0; c < 3; ++c) m |= (fs[3 * i + c] ? 1 : 0) << c;
     hfix[d] = m;
   }
   for (long long d = 0; d < NN; ++d) nF += hfix[d] ? 1 : 0;
   if (nF == 0) { P.err = "No Fixations!"; return T3X_ERR_NO_FIX; }
+  P.hLoaded = loaded;
 
   T3X_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   // ---- device arrays per level
@@ -1469,31 +1475,137 @@ int t3x_stream_copy(long long bytes, int reps, double* msPerCopy) {
   return rc;
 }
 
+namespace {
+
+// Uploads the passive cells (h->hPassive) and the design cells (solid and not
+// passive, 274-276).
+int apply_passive(t3x_problem* h) {
+  const Level& F = h->lv[0];
+  std::vector<uint8_t> act(F.hEmask);
+  int cnt = 0;
+  for (long long d = 0; d < F.NN; ++d)
+    if (h->hPassive[d]) {
+      act[d] = 0;
+      ++cnt;
+    }
+  if (!h->passive) {
+    h->passive = h->alloc<uint8_t>(F.NN);
+    h->active = h->alloc<uint8_t>(F.NN);
+  }
+  h->upload(h->passive, h->hPassive.data(), F.NN);
+  h->upload(h->active, act.data(), F.NN);
+  h->nPassive = cnt;
+  return T3X_OK;
+}
+
+// One layer of Common_IncludeAdjacentElements (1587-1615): the 3x3x3
+// neighbourhood of every listed cell, clipped to the grid and to solid cells.
+std::vector<uint8_t> dilate26(const Level& F, const std::vector<uint8_t>& in) {
+  std::vector<uint8_t> out(F.NN, 0);
+  for (int iz = 0; iz < F.nz; ++iz)
+    for (int ix = 0; ix < F.nx; ++ix)
+      for (int iy = 0; iy < F.ny; ++iy) {
+        const long long d = iy + (long long)F.NY * (ix + (long long)F.NX * iz);
+        if (!in[d]) continue;
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy) {
+              const int y = iy + dy, x = ix + dx, z = iz + dz;
+              if (y < 0 || x < 0 || z < 0 || y >= F.ny || x >= F.nx || z >= F.nz) continue;
+              const long long e = y + (long long)F.NY * (x + (long long)F.NX * z);
+              if (F.hEmask[e]) out[e] = 1;
+            }
+      }
+  return out;
+}
+
+}  // namespace
+
 int t3x_set_passive(t3x_problem* h, int n, const int* voxelIds) {
   return guarded(h, [&] {
     if (n < 0 || (n > 0 && !voxelIds)) { h->err = "t3x_set_passive: bad arguments"; return (int)T3X_ERR_INPUT; }
     const Level& F = h->lv[0];
-    std::vector<uint8_t> pas(F.NN, 0), act(F.hEmask);
-    int cnt = 0;
+    h->hPassive.assign(F.NN, 0);
     for (int i = 0; i < n; ++i) {
       const long long v = voxelIds[i];
       if (v < 0 || v >= (long long)h->nely * h->nelx * h->nelz) continue;
       const int iy = (int)(v % h->nely), ix = (int)((v / h->nely) % h->nelx), iz = (int)(v / ((long long)h->nely * h->nelx));
       const long long d = iy + (long long)F.NY * (ix + (long long)F.NX * iz);
-      if (!F.hEmask[d] || pas[d]) continue;  // void voxels are not elements (527)
-      pas[d] = 1;
-      act[d] = 0;
-      ++cnt;
+      if (F.hEmask[d]) h->hPassive[d] = 1;  // void voxels are not elements (527)
     }
-    if (!h->passive) {
-      h->passive = h->alloc<uint8_t>(F.NN);
-      h->active = h->alloc<uint8_t>(F.NN);
-    }
-    h->upload(h->passive, pas.data(), F.NN);
-    h->upload(h->active, act.data(), F.NN);
-    h->nPassive = cnt;
-    return (int)T3X_OK;
+    return apply_passive(h);
   });
+}
+
+int t3x_set_passive_layers(t3x_problem* h, int numLayerBoundary, int numLayerLoads, int numLayerFixation) {
+  return guarded(h, [&] {
+    const Level& F = h->lv[0];
+    const long long NN = F.NN;
+    if (h->hPassive.size() != (size_t)NN) h->hPassive.assign(NN, 0);
+    // elementsOnBoundary (1714-1728): solid cells with a node shared by fewer
+    // than 8 solid cells
+    std::vector<uint8_t> bnode(NN, 0), bcell(NN, 0);
+    for (int iz = 0; iz < F.NZ; ++iz)
+      for (int ix = 0; ix < F.NX; ++ix)
+        for (int iy = 0; iy < F.NY; ++iy) {
+          const long long d = iy + (long long)F.NY * (ix + (long long)F.NX * iz);
+          if (!F.hNmask[d]) continue;
+          int k = 0;
+          for (int c = 0; c < 8; ++c) {
+            const int y = iy - 1 + (c & 1), x = ix - 1 + ((c >> 1) & 1), z = iz - 1 + (c >> 2);
+            if (y < 0 || x < 0 || z < 0 || y >= F.ny || x >= F.nx || z >= F.nz) continue;
+            k += F.hEmask[y + (long long)F.NY * (x + (long long)F.NX * z)] ? 1 : 0;
+          }
+          bnode[d] = k < 8;
+        }
+    auto cell_touches = [&](long long d, const std::vector<uint8_t>& nodes) {
+      for (int j = 0; j < 8; ++j)
+        if (nodes[d + kLocDy[j] + (long long)F.NY * (kLocDx[j] + (long long)F.NX * kLocDz[j])]) return true;
+      return false;
+    };
+    for (int iz = 0; iz < F.nz; ++iz)
+      for (int ix = 0; ix < F.nx; ++ix)
+        for (int iy = 0; iy < F.ny; ++iy) {
+          const long long d = iy + (long long)F.NY * (ix + (long long)F.NX * iz);
+          bcell[d] = F.hEmask[d] && cell_touches(d, bnode);
+        }
+    // layer 1 as given, every further layer one neighbourhood step (1521-1575)
+    auto layers = [&](std::vector<uint8_t> set, int n) {
+      for (int k = 2; k <= n; ++k) set = dilate26(F, set);
+      return set;
+    };
+    auto merge = [&](const std::vector<uint8_t>& set) {
+      for (long long d = 0; d < NN; ++d) h->hPassive[d] |= set[d];
+    };
+    if (numLayerBoundary > 0) merge(layers(bcell, numLayerBoundary));
+    // elements near loads / fixations: the boundary cells around the loaded /
+    // fixed nodes (nodeStruct_ of 1536-1546 lists boundary cells only)
+    for (int pass = 0; pass < 2; ++pass) {
+      const int n = pass == 0 ? numLayerLoads : numLayerFixation;
+      if (n <= 0) continue;
+      const std::vector<uint8_t>& nodes = pass == 0 ? h->hLoaded : h->hFixNode;
+      std::vector<uint8_t> set(NN, 0);
+      for (long long d = 0; d < NN; ++d) set[d] = bcell[d] && cell_touches(d, nodes);
+      merge(layers(set, n));
+    }
+    return apply_passive(h);
+  });
+}
+
+int t3x_get_passive(const t3x_problem* h, int cap, int* voxelIds, int* count) {
+  if (!h || !count || h->lv.empty()) return T3X_ERR_INPUT;
+  const Level& F = h->lv[0];
+  int n = 0;
+  if (h->hPassive.size() == (size_t)F.NN)
+    for (int iz = 0; iz < h->nelz; ++iz)
+      for (int ix = 0; ix < h->nelx; ++ix)
+        for (int iy = 0; iy < h->nely; ++iy) {
+          if (!h->hPassive[iy + (long long)F.NY * (ix + (long long)F.NX * iz)]) continue;
+          if (voxelIds && n < cap) voxelIds[n] = iy + h->nely * (ix + h->nelx * iz);
+          ++n;
+        }
+  *count = n;
+  return T3X_OK;
 }
 
 int t3x_memory(const t3x_problem* h, long long* bytes) {

@@ -107,6 +107,8 @@ def load_library():
                                 C.POINTER(C.c_longlong)]
     L.t3x_memory.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.t3x_set_passive.argtypes = [vp, C.c_int, C.c_void_p]
+    L.t3x_set_passive_layers.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.t3x_get_passive.argtypes = [vp, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
     L.t3x_stream_copy.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.t3x_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(vp)]
     L.t3x_server_submit.argtypes = [vp, C.POINTER(Request)]
@@ -123,6 +125,7 @@ EXPORTED = ["t3x_default_params", "t3x_create", "t3x_create_multi", "t3x_num_loa
             "t3x_opt_begin", "t3x_opt_step", "t3x_opt_result", "t3x_kernel_time", "t3x_kernel_time_reset",
             "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance",
             "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy", "t3x_set_passive",
+            "t3x_set_passive_layers", "t3x_get_passive",
             "t3x_server_create", "t3x_server_submit", "t3x_server_poll", "t3x_server_release",
             "t3x_server_stats", "t3x_server_destroy"]
 
@@ -284,6 +287,19 @@ class Problem:
         ids = np.ascontiguousarray(voxel_ids, dtype=np.int32)
         self._keep.append(ids)
         self._check(self.L.t3x_set_passive(self.h, len(ids), ids.ctypes.data if len(ids) else None))
+
+    def set_passive_layers(self, n_boundary, n_loads, n_fixations):
+        """specifyPassiveRegions_ (37): TopOpti_SetPassiveElements (t3x_set_passive_layers)."""
+        self._check(self.L.t3x_set_passive_layers(self.h, int(round(n_boundary)), int(round(n_loads)),
+                                                  int(round(n_fixations))))
+
+    def passive(self):
+        """Passive voxel ids, 0-based on the unpadded grid (t3x_get_passive)."""
+        n = C.c_int(0)
+        self._check(self.L.t3x_get_passive(self.h, 0, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._check(self.L.t3x_get_passive(self.h, n.value, out.ctypes.data, C.byref(n)))
+        return out[:n.value]
 
     def memory(self):
         """Device bytes the problem holds (t3x_memory)."""
