@@ -4,8 +4,9 @@
 %%   TOP3D_XL(true(50,100,50), 'GLOBAL', 0.12, 50, sqrt(3));
 %%   TOP3D_XL(true(50,100,50), 'LOCAL', 0.5, 300, sqrt(3), 6);
 %%   TOP3D_XL('./data/Femur.TopVoxel', 'GLOBAL', 0.4, 50, sqrt(3));
-%% Outputs as in the reference: ./out/{iters,c,sharp,timing}_Target.dat and
-%% the global densityLayout_ (xPhys over the solid voxels).
+%% Outputs as in the reference: ./out/{iters,c,sharp,timing}_Target.dat,
+%% DesignVolume.{nii,stl} and the global densityLayout_ (xPhys over the
+%% solid voxels).
 function TOP3D_XL(inputModel, consType, V0, nLoop, rMin, varargin)
 	global densityLayout_;
 	switch consType
@@ -17,11 +18,11 @@ function TOP3D_XL(inputModel, consType, V0, nLoop, rMin, varargin)
 	end
 	params = struct('precision', 0); %% 0: FP64 plain path (1: mixed precision)
 	if ischar(inputModel) || isstring(inputModel)
-		M = ReadTopVoxel(inputModel);
+		M = ReadTopVoxel(inputModel); voxels = M.voxels;
 		h = t3x_mex('create', M.voxels, M.fix(:,1), M.fix(:,2:4), M.load(:,1), M.load(:,2:4), params, M.extraLoads);
 		if ~isempty(M.passive), t3x_mex('passive', h, M.passive); end
 	elseif islogical(inputModel)
-		h = t3x_mex('create', inputModel, params);
+		h = t3x_mex('create', inputModel, params); voxels = inputModel;
 	else
 		error('Inappropriate Input Model!');
 	end
@@ -37,6 +38,15 @@ function TOP3D_XL(inputModel, consType, V0, nLoop, rMin, varargin)
 	WriteColumn(strcat(outPath, 'c_Target.dat'), '%30.16e\n', out.cHist);
 	WriteColumn(strcat(outPath, 'sharp_Target.dat'), '%30.16e\n', out.sharpHist);
 	WriteColumn(strcat(outPath, 'timing_Target.dat'), '%16.6e\n', out.tHist(:,end));
+	%% design export (206-221): the density on the padded grid, flipped in y,
+	%% closed by its caps at the 0.5 level
+	res = t3x_mex('resolution', h);
+	[iy, ix, iz] = ind2sub(size(voxels), find(voxels)); %% compact cell order = column-major voxel order
+	vol = zeros(res); vol(sub2ind(res, iy, ix, iz)) = out.xPhys; vol = flip(vol, 1);
+	fs = isosurface(vol, 0.5); fc = isocaps(vol, 0.5);
+	tri = triangulation([fs.faces; size(fs.vertices,1) + fc.faces], [fs.vertices; fc.vertices]);
+	stlwrite(tri, strcat(outPath, 'DesignVolume.stl'));
+	niftiwrite(flip(vol, 1), strcat(outPath, 'DesignVolume.nii'));
 end
 
 function WriteColumn(fileName, fmt, v)

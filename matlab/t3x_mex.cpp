@@ -15,6 +15,7 @@
 //                                                      a cell array of n x 4 [node Fx Fy Fz] matrices
 //   t3x_mex('passive', h, voxelIds)                   passive elements, 1-based voxel ids
 //   out = t3x_mex('run', h, consType, V0, nLoop, rMin [, rHat])
+//   res = t3x_mex('resolution', h)                    padded cell grid [resY resX resZ]
 //   t3x_mex('destroy', h)
 //
 // Status codes become the reference's errors and warnings:
@@ -225,6 +226,16 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     for (size_t i = 0; i < n; ++i) ids[i] = (int)v[i] - 1;  // 1-based -> 0-based
     const int rc = t3x_set_passive(h, (int)n, ids.data());
     if (rc != T3X_OK) fail(rc, h);
+  } else if (!std::strcmp(cmd, "resolution")) {
+    // [resY resX resZ]: the padded cell grid (1651-1671), for the design export
+    if (nrhs < 2) mexErrMsgIdAndTxt("t3x:input", "Wrong Input!");
+    t3x_problem* h = handle_arg(prhs[1]);
+    int info[1 + 8 * 16];
+    t3x_info(h, info);
+    const double r[3] = {(double)info[2], (double)info[1], (double)info[3]};
+    mxArray* a = mxCreateDoubleMatrix(1, 3, mxREAL);
+    std::memcpy(mxGetDoubles(a), r, sizeof(r));
+    plhs[0] = a;
   } else if (!std::strcmp(cmd, "destroy")) {
     if (nrhs < 2) mexErrMsgIdAndTxt("t3x:input", "Wrong Input!");
     t3x_problem* h = handle_arg(prhs[1]);

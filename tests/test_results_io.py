@@ -1,5 +1,6 @@
 """CPU tests of the result/trajectory writers (TOP3D_XL.m:187-204, 410-421, 564-570)."""
 import numpy as np
+import pytest
 
 from oracle import Oracle
 from top3d_xl_amd.irregular import generate
@@ -32,3 +33,22 @@ def test_target_files_and_nii(tmp_path):
     write_nii(str(tmp_path / "d.nii"), V)
     assert (tmp_path / "d.nii").stat().st_size == 352 + 8 * V.size
     assert np.array_equal(read_nii(str(tmp_path / "d.nii")), V)
+
+
+def test_design_outputs_written(tmp_path):
+    from top3d_xl_amd.isosurface import enclosed_volume, read_stl
+    from top3d_xl_amd.results import write_design_outputs
+    vox = np.ones((4, 6, 4), bool)
+    res = (4, 8, 4)  # padded in x
+    r = dict(xPhys=np.ones(vox.sum()), cHist=[1.0, 0.5], sharpHist=[0.2, 0.1], lssIts=[7, 8],
+             tHist=np.ones((2, 6)))
+    write_design_outputs(str(tmp_path), r, vox, res)
+    for f in ("iters_Target.dat", "c_Target.dat", "sharp_Target.dat", "timing_Target.dat", "DesignVolume.nii",
+              "DesignVolume.stl"):
+        assert (tmp_path / f).exists()
+    V = read_nii(str(tmp_path / "DesignVolume.nii"))
+    assert V.shape == res and V[:, :6].min() == 1.0 and V[:, 6:].max() == 0.0
+    tris, _ = read_stl(str(tmp_path / "DesignVolume.stl"))
+    # all-solid 4x6x4 cells: samples 1 on 4x6x4 grid points, 0 beyond x = 6:
+    # the box [1,6.5] x [1,4] x [1,4] (the level crosses halfway to the void column)
+    assert enclosed_volume(tris) == pytest.approx(5.5 * 3 * 3, rel=1e-6)

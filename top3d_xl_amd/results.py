@@ -4,6 +4,8 @@
 * c_Target.dat      : compliance history, '%30.16e\\n' (196-198, 413-415)
 * sharp_Target.dat  : sharpness history, '%30.16e\\n' (199-201, 416-418)
 * timing_Target.dat : total time per design iteration, '%16.6e\\n' (202-204, 419-421)
+* DesignVolume.stl  : closed 0.5 iso-surface of that volume, flipped along y
+                      (206-222, 572-577; see isosurface.py)
 * DesignVolume.nii  : xPhys scattered into the padded (resY, resX, resZ) grid,
                       zeros elsewhere, as a NIfTI-1 float64 volume
                       (IO_ExportDesignInVolume_nii, 564-570; MATLAB's
@@ -76,3 +78,14 @@ def read_nii(path):
     off = int(struct.unpack_from("<f", raw, 108)[0])
     n = int(np.prod(dims))
     return np.frombuffer(raw, dtype="<f8", count=n, offset=off).reshape(dims, order="F")
+
+
+def write_design_outputs(out_dir, result, voxels, res):
+    """Everything a design run leaves in outPath: the *_Target.dat files
+    (193-204), DesignVolume.nii (185-186, 564-570) and DesignVolume.stl
+    (206-221).  res is the padded (resY, resX, resZ) cell grid."""
+    from .isosurface import write_design_stl
+    write_target_files(out_dir, result)
+    cm = cell_map_from_voxels(voxels, res)
+    write_nii(os.path.join(out_dir, "DesignVolume.nii"), padded_volume(result["xPhys"], cm, res))
+    write_design_stl(os.path.join(out_dir, "DesignVolume.stl"), result["xPhys"], cm, res)

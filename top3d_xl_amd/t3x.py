@@ -335,7 +335,8 @@ def TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin, **params):
     """Drop-in for TOP3D_XL.m:4-19.  inputModel is a logical (nely,nelx,nelz)
     array (the built-in cuboid BCs apply, 529-560) or a (voxels, fix, load)
     or (voxels, fix, load, extra_loads) tuple for explicit boundary
-    conditions and further load cases."""
+    conditions and further load cases.  out_dir=path also writes the
+    reference's output files there (*_Target.dat, DesignVolume.nii/.stl)."""
     extra = ()
     if isinstance(inputModel, tuple):
         vox, fix, load = inputModel[:3]
@@ -353,9 +354,15 @@ def TOP3D_XL(inputModel, consType, V0, nLoop, rMin, *varargin, **params):
         mode = 1
     else:
         raise T3XError("Wrong Input!")
+    out_dir = params.pop("out_dir", None)
     P = Problem(vox, fix, load, extra_loads=extra, **params)
     try:
-        return P.run(mode, V0, nLoop, rMin, rHat)
+        r = P.run(mode, V0, nLoop, rMin, rHat)
+        if out_dir is not None:  # outPath (186-221 / 402-440)
+            from .results import write_design_outputs
+            lv = P.levels[0]
+            write_design_outputs(out_dir, r, vox, (lv["resY"], lv["resX"], lv["resZ"]))
+        return r
     finally:
         P.close()
 
