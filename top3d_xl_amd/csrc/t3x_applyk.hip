@@ -2,10 +2,12 @@
 //
 // Node-centric and z-marching: a work-group owns a 32x8 tile of node columns
 // (y, x) and walks a chunk of node layers in z.  The masked displacements of
-// the node layers (tile + 1-node halo, SoA, y fastest) and the cell moduli
-// sit in three LDS buffers: while cell layer k is computed from node layers
-// k and k+1, the global loads of layer k+2 are in flight into registers and
-// land in the third buffer afterwards, so one barrier per layer suffices.
+// the node layers (tile + 1-node halo, SoA, y fastest) sit in three LDS
+// buffers: while cell layer k is computed from node layers k and k+1, the
+// global loads of layer k+2 are in flight into registers and land in the
+// third buffer afterwards, so one barrier per layer suffices.  The moduli of
+// the 4 cells around a thread's column come straight from global memory
+// (cache hits), one layer ahead.
 // Each thread adds, for the 4 cells of layer k around its column, the rows
 // of E_e*Ke belonging to its node in layer k (which then is complete and is
 // written) and to its node in layer k+1 (carried to the next step), reading
@@ -27,7 +29,6 @@ constexpr int AK_TY = 32, AK_TX = 8, AK_T = AK_TY * AK_TX;  // 256 threads
 constexpr int AK_HY = AK_TY + 2, AK_HX = AK_TX + 2;         // halo tile
 constexpr int AK_ZC = 32;                                   // node layers per work-group (minimum)
 constexpr int AK_NS = (AK_HX * AK_HY + AK_T - 1) / AK_T;    // staged halo nodes per thread (2)
-constexpr int AK_EY = AK_TY + 1, AK_EX = AK_TX + 1;         // staged cells per tile edge
 
 // Ke through a constant-address-space pointer: its entries are wave-uniform
 // scalar loads.
@@ -83,7 +84,6 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, int zc, const T* 
                                                         double omega) {
   if (fl && fl[FL_DONE]) return;
   __shared__ T sU[3][3 * AK_HX * AK_HY];
-  __shared__ T sE[3][AK_EX * AK_EY];
   const int tid = threadIdx.x;
   const int ty = tid % AK_TY, tx = tid / AK_TY;
   const int y0 = blockIdx.x * AK_TY, x0 = blockIdx.y * AK_TX;
@@ -112,8 +112,8 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, int zc, const T* 
       }
     }
   };
-  // LDS stores of node layer z (from registers) and of cell layer z's moduli
-  auto store_stage = [&](int z, int b) {
+  // LDS stores of node layer z (from registers)
+  auto store_stage = [&](int b) {
 #pragma unroll
     for (int s2 = 0; s2 < AK_NS; ++s2) {
       const int t = tid + s2 * AK_T;
@@ -123,13 +123,16 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, int zc, const T* 
         for (int c = 0; c < 3; ++c) sU[b][(c * AK_HX + hx) * AK_HY + hy] = pu[s2][c];
       }
     }
-    for (int t = tid; t < AK_EX * AK_EY; t += AK_T) {
-      const int ey = t % AK_EY, ex = t / AK_EY;
-      const int gy = y0 - 1 + ey, gx = x0 - 1 + ex;
-      T e = 0;
-      if (z >= 0 && z < g.nz && gy >= 0 && gx >= 0 && gy < g.ny && gx < g.nx)
-        e = T(E[gy + (long long)g.NY * gx + plane * z]);
-      sE[b][t] = e;
+  };
+  // moduli of the 4 cells of layer z around this column: (y-1+CY, x-1+CX)
+  T ep[4];
+  auto load_moduli = [&](int z) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int cy = yy - 1 + (c & 1), cx = xx - 1 + (c >> 1);
+      ep[c] = (z >= 0 && z < g.nz && cy >= 0 && cx >= 0 && cy < g.ny && cx < g.nx)
+                  ? T(E[cy + (long long)g.NY * cx + plane * z])
+                  : T(0);
     }
   };
   auto buf = [](int z) { return (z + 3) % 3; };  // z >= -1
@@ -138,19 +141,19 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, int zc, const T* 
   double pq = 0.0;
   // cell layers k = zc0-1 .. zc1-1 from node layers k (buffer buf(k)) and k+1
   const int kBeg = zc0 - 1;
+  load_moduli(kBeg);
   load_stage(kBeg);
-  store_stage(kBeg, buf(kBeg));
+  store_stage(buf(kBeg));
   load_stage(kBeg + 1);
-  store_stage(kBeg + 1, buf(kBeg + 1));
+  store_stage(buf(kBeg + 1));
   __syncthreads();
   for (int k = kBeg; k < zc1; ++k) {
     const bool more = k + 2 <= zc1;
     if (more) load_stage(k + 2);  // in flight during this layer's FMAs
     const T* s0 = sU[buf(k)];
     const T* s1 = sU[buf(k + 1)];
-    const T* se = sE[buf(k)];
-    const T e00 = se[tx * AK_EY + ty], e10 = se[tx * AK_EY + ty + 1];
-    const T e01 = se[(tx + 1) * AK_EY + ty], e11 = se[(tx + 1) * AK_EY + ty + 1];
+    const T e00 = ep[0], e10 = ep[1], e01 = ep[2], e11 = ep[3];
+    load_moduli(k + 1);  // in flight as well
     // cells (y-1+CY, x-1+CX) of layer k.  The Ke pointer is made opaque per
     // layer so its 576 entries are loaded where they are used instead of
     // being hoisted out of the loop (1152 SGPRs, spilled to VGPR lanes).
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(AK_T) void k_apply_K_tiled(Geo g, int zc, const T* 
     if (e11 != T(0)) cell_pair<T, 1, 1>(Kq, s0, s1, tx, ty, e11, a0, a1);
     // the third buffer held node layer k-1, which no thread reads any more
     // (every thread passed the previous layer's barrier)
-    if (more) store_stage(k + 2, buf(k + 2));
+    if (more) store_stage(buf(k + 2));
     if (k >= zc0 && own) {  // node layer k complete
       const long long n = yy + (long long)g.NY * xx + plane * k;
       const uint8_t f = fix[n];
