@@ -11,7 +11,8 @@ By default the precision form and the FP64 plain path run side by side on
 the same workload.  The precision form is the reference's PCG in FP64 (K*p,
 vectors, dot products) preconditioned in FP32 by the 'Standard' V(1,1) cycle
 (damped Jacobi, omega 0.4, K*u on every level) around the FP64 coarsest
-Cholesky; the FP32 'Adapted' cycle is run as well.  `value` is the precision
+Cholesky; the FP32 'Adapted' cycle and the 'Standard' cycle in FP64 are run as
+well.  `value` is the precision
 form's ms per design iteration; the line carries the FP64 figures, the MG-PCG
 speed-up and the per-iteration compliance error against the FP64 path.
 
@@ -41,8 +42,10 @@ def parse_size(s):
     return ny, nx, nz
 
 
+# fp64_standard: the precision form's cycle (and omega) in FP64, which
+# separates what the FP32 arithmetic gains from what the cycle gains
 VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=1, vcycle=1, omega=0.4),
-            "mixed_adapted": dict(precision=1)}
+            "mixed_adapted": dict(precision=1), "fp64_standard": dict(precision=0, vcycle=1, omega=0.4)}
 
 
 def make_model(args):
@@ -141,7 +144,8 @@ def main():
         args.size = "256x512x256" if args.workload == "cuboid" else "384x768x384"
     model = make_model(args)
     runs = {}
-    names = {"fp64": ["fp64"], "mixed": ["mixed"], "both": ["fp64", "mixed", "mixed_adapted"]}[args.precision]
+    names = {"fp64": ["fp64"], "mixed": ["mixed"],
+             "both": ["fp64", "mixed", "mixed_adapted", "fp64_standard"]}[args.precision]
     for name in names:
         print(f"[bench] {name}: {VARIANTS[name]}", file=sys.stderr, flush=True)
         runs[name] = gpu_run(args, VARIANTS[name], model)
@@ -219,6 +223,9 @@ def main():
             out[f"mg_pcg_speedup_{k}_vs_fp64"] = round((a["cg_s"] / a["steps"]) / (b["cg_s"] / b["steps"]), 3)
             out[f"iteration_speedup_{k}_vs_fp64"] = round(a["ms_per_step"] / b["ms_per_step"], 3)
             out[f"compliance_rel_err_{k}_vs_fp64"] = [abs(x - y) / abs(y) for x, y in zip(b["c_all"], a["c_all"])]
+    if "fp64_standard" in runs and "mixed" in runs:
+        a, b = runs["fp64_standard"], runs["mixed"]
+        out["mg_pcg_speedup_mixed_vs_fp64_standard"] = round((a["cg_s"] / a["steps"]) / (b["cg_s"] / b["steps"]), 3)
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))
