@@ -48,6 +48,25 @@ VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=1, vcycle=1, omeg
             "mixed_adapted": dict(precision=1), "fp64_standard": dict(precision=0, vcycle=1, omega=0.4)}
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_fp64_256x512x256.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (fetched + written) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the 256x512x256 FP64 run
+    (counter values scaled by the copy-kernel calibration stated there), or
+    None when the summary has no row for it."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                cols = [c.strip() for c in line.split("|")]
+                if len(cols) > 5 and kernel in cols[1]:
+                    return round((float(cols[4]) + float(cols[5])) * 1e6)
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def make_model(args):
     """The workload's voxels and boundary conditions (None: built-in cuboid BCs)."""
     import numpy as np
@@ -175,7 +194,11 @@ def main():
     gbs = bytes_ / k_avg_s / 1e9
     peak = FP64_PEAK_TFLOPS
     roofline = {"bound": "mfma", "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(tflops / peak, 4), "traffic": None,
+                "frac": round(tflops / peak, 4),
+                # measured on this workload only (rocprofv3 PMC runs, committed summary)
+                "traffic": measured_traffic("k_apply_K_tiled")
+                if args.workload == "cuboid" and args.size == "256x512x256" else None,
+                "traffic_source": "profiles/traffic_fp64_256x512x256.md (FETCH_SIZE + WRITE_SIZE, bytes per launch)",
                 "kernel": "k_apply_K_tiled (fine-level K*p inside PCG, %s)" % head, "launches": kn,
                 "avg_ms": round(1e3 * k_avg_s, 4), "algorithmic_flop": flops,
                 "algorithmic_bytes": bytes_, "achieved_GBs_algorithmic": round(gbs, 1)}
