@@ -65,3 +65,33 @@ def test_footprint_estimate_scales():
     assert 6 * a < b < 10 * a  # ~8x the nodes
     assert t3x.footprint(64, 128, 64, mode="LOCAL") > b  # MMA state
     assert t3x.footprint(64, 128, 64, precision=1) > b  # FP32 V-cycle copies
+
+
+def test_create_multi_checks_arguments_before_device():
+    """t3x_create_multi refuses null or negative boundary-condition arrays with
+    T3X_ERR_INPUT before it looks for a device."""
+    import ctypes as C
+    import top3d_xl_amd as t3x
+    L = t3x.load_library()
+    vox = np.ones(4 * 8 * 4, dtype=np.uint8)
+    P = t3x.default_params()
+    h = C.c_void_p()
+    fn = np.array([0, 1], dtype=np.int32)
+    fs = np.ones(6, dtype=np.uint8)
+    npc = np.array([1, -1], dtype=np.int32)
+    ln = np.array([5], dtype=np.int32)
+    lv = np.zeros(3)
+    cases = [
+        (2, None, fs.ctypes.data, 1, np.array([1], np.int32).ctypes.data, ln.ctypes.data, lv.ctypes.data),
+        (2, fn.ctypes.data, fs.ctypes.data, 2, npc.ctypes.data, ln.ctypes.data, lv.ctypes.data),
+        (2, fn.ctypes.data, fs.ctypes.data, 1, np.array([1], np.int32).ctypes.data, None, lv.ctypes.data),
+        (2, fn.ctypes.data, fs.ctypes.data, 1, None, ln.ctypes.data, lv.ctypes.data),
+    ]
+    keep = [np.array([1], np.int32) for _ in cases]
+    for (nf, f, s, nc, pc, l, v), k in zip(cases, keep):
+        if pc is not None and nc == 1:
+            pc = k.ctypes.data
+        rc = L.t3x_create_multi(vox.ctypes.data, 4, 8, 4, nf, f, s, nc, pc, l, v, C.byref(P), C.byref(h))
+        assert rc == 1, rc  # T3X_ERR_INPUT
+        assert b"boundary-condition" in L.t3x_last_error(h)
+        L.t3x_destroy(h)

@@ -1349,9 +1349,22 @@ int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nF
     *out = P.release();
     return T3X_ERR_INPUT;
   }
+  if (nFix >= 0) {  // explicit boundary conditions: the arrays must be there
+    long long nl = 0;
+    bool ok = (nFix == 0 || (fixNodes && fixState)) && nCases >= 0 && (nCases == 0 || nPerCase);
+    for (int ii = 0; ok && ii < nCases; ++ii) {
+      ok = nPerCase[ii] >= 0;
+      nl += nPerCase[ii];
+    }
+    if (!ok || (nl > 0 && (!loadNodes || !loadVec))) {
+      P->err = "t3x_create: null or negative boundary-condition arguments";
+      *out = P.release();
+      return T3X_ERR_INPUT;
+    }
+  }
   int rc;
   try {
-    if (nFix >= 0 && (nCases < 1 || !nPerCase)) {
+    if (nFix >= 0 && nCases < 1) {
       P->err = "No Loads!";
       rc = T3X_ERR_NO_LOADS;
     } else {
