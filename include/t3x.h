@@ -57,6 +57,10 @@ typedef struct t3x_params {
                             coarse levels through the stored Galerkin matrices,
                             727-744): FP64 with precision = 0, FP32 level
                             vectors and matrices with precision = 1 */
+  int graphs;            /* 1 = each PCG iteration is captured once as a HIP graph and
+                            replayed (not while timing = 1); 0 (default) =
+                            kernel-by-kernel launches.  Results are identical
+                            either way; on MI355X the speed is too. */
 } t3x_params;
 
 /* Fills the reference defaults (InitialSettings, 21-38). */

@@ -72,7 +72,7 @@ void read_params(const mxArray* s, t3x_params* p) {
   num("E0", &p->E0); num("nu", &p->nu); num("Emin", &p->Emin); num("penal", &p->penal); num("tol", &p->tol);
   inum("maxIT", &p->maxIT); num("omega", &p->omega); inum("coarsestControl", &p->coarsestControl);
   inum("precision", &p->precision); inum("checkEvery", &p->checkEvery); inum("timing", &p->timing);
-  inum("vcycle", &p->vcycle);
+  inum("vcycle", &p->vcycle); inum("graphs", &p->graphs);
 }
 
 mxArray* dvec(const double* v, int n) {

@@ -30,7 +30,8 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 class Params(C.Structure):
     _fields_ = [("E0", C.c_double), ("nu", C.c_double), ("Emin", C.c_double), ("penal", C.c_double),
                 ("tol", C.c_double), ("maxIT", C.c_int), ("omega", C.c_double), ("coarsestControl", C.c_int),
-                ("precision", C.c_int), ("checkEvery", C.c_int), ("timing", C.c_int), ("vcycle", C.c_int)]
+                ("precision", C.c_int), ("checkEvery", C.c_int), ("timing", C.c_int), ("vcycle", C.c_int),
+                ("graphs", C.c_int)]
 
 
 class Iter(C.Structure):
