@@ -77,6 +77,13 @@ void t3x_default_params(t3x_params* p);
  *   (nely+1)x(nelx+1)x(nelz+1) grid (the .TopVoxel convention, 474-481,
  *   adapted as in AdaptBCExternalMdl 1439-1489), fixState nFix*3 flags,
  *   loadVec nLoad*3 forces.
+ * Every listed fixed DOF is fixed (P_free K P_free + I on the fixed rows).
+ * The reference applies a fixation only at boundary nodes (it intersects
+ * fixingCond_ with nodesOnBoundary, 1369) and then pairs the ii-th common
+ * node's elements with fixingCond_'s ii-th row (1379-1383), so a fixation
+ * at a node inside the domain is dropped and misaligns the later ones; for
+ * fixations on the boundary (supports on the surface, the built-in cuboid)
+ * the two agree.
  */
 int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
                const uint8_t* fixState, int nLoad, const int* loadNodes, const double* loadVec,

@@ -69,6 +69,7 @@ def _load():
     L.orc_set_passive_layers.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.orc_set_vcycle.argtypes = [vp, C.c_int]
     L.orc_set_omega.argtypes = [vp, C.c_double]
+    L.orc_set_tol.argtypes = [vp, C.c_double, C.c_int]
     L.orc_apply_K_level.argtypes = [vp, C.c_int, _dp, _dp]
     _lib = L
     return L
@@ -161,6 +162,10 @@ class Oracle:
         """typeVcycle_ (34): 'Adapted' or 'Standard'."""
         if self.L.orc_set_vcycle(self.h, {"Adapted": 0, "Standard": 1}[kind]) != 0:
             raise ValueError(kind)
+
+    def set_tol(self, tol, max_it=800):
+        """tol_ (30) and maxIT_ (31) of the elasticity solves."""
+        self.L.orc_set_tol(self.h, float(tol), int(max_it))
 
     def set_omega(self, w):
         """weightFactorJacobi_ (32)."""

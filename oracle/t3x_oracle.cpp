@@ -523,7 +523,7 @@ bool apply_bc(Model& M) {
 }
 
 // Solving_SetupKeWithFixedDOFs + Solving_ApplyBConEleStiffMat_B, 1338-1437.
-void setup_bc_ke(Model& M) {
+bool setup_bc_ke(Model& M) {
   const Level& L = M.L[0];
   std::vector<char> isBE(L.numElements, 0);
   for (int e : L.elementsOnBoundary) isBE[e] = 1;
@@ -537,12 +537,19 @@ void setup_bc_ke(Model& M) {
         int n = L.eNodMat[(size_t)e * 8 + j];
         if (onB[n]) nodeEles[n].push_back(e);
       }
-  // 1369-1374.  Fixed nodes must be boundary nodes (true for every BC set
-  // this library builds; the reference pairs intersect() output with
-  // fixingCond_ rows, which is only consistent in that case).
+  // 1369-1374.  Fixed nodes must be boundary nodes: the reference intersects
+  // fixingCond_ with nodesOnBoundary but then pairs the ii-th common node's
+  // element list with fixingCond_'s ii-th row (1379-1383), so a fixation
+  // inside the domain both goes missing and shifts every later pairing.
+  // That case is refused here rather than restated.
   std::vector<int> allFixedEles;
-  for (int n : M.fixNode)
+  for (int n : M.fixNode) {
+    if (nodeEles[n].empty()) {
+      M.err = "fixation at a node inside the domain (not on its boundary): not restated";
+      return false;
+    }
     for (int e : nodeEles[n]) allFixedEles.push_back(e);
+  }
   std::sort(allFixedEles.begin(), allFixedEles.end());
   allFixedEles.erase(std::unique(allFixedEles.begin(), allFixedEles.end()), allFixedEles.end());
   M.mapUniqueKes.assign(L.numElements, 0);
@@ -587,6 +594,7 @@ void setup_bc_ke(Model& M) {
       M.l2BC[jj] = (int)M.sonElesWithFixed.size();
     }
   }
+  return true;
 }
 
 // ------------------------------------------------------- Galerkin assembly
@@ -1618,7 +1626,7 @@ void* orc_create_multi(const unsigned char* voxels, int nely, int nelx, int nelz
     }
   if (!apply_bc(*M)) return M;
   build_hierarchy(*M);
-  setup_bc_ke(*M);
+  if (!setup_bc_ke(*M)) return M;
   M->bcOk = true;
   return M;
 }
@@ -1700,6 +1708,12 @@ int orc_vcycle(void* h, const double* r, double* x) { precondition(*(Model*)h, r
 int orc_set_vcycle(void* h, int type) {
   if (type != 0 && type != 1) return 1;
   ((Model*)h)->S.typeVcycle = type;
+  return 0;
+}
+// tol_ (30) and maxIT_ (31) of the elasticity solves.
+int orc_set_tol(void* h, double tol, int maxIT) {
+  ((Model*)h)->S.tol = tol;
+  ((Model*)h)->S.maxIT = maxIT;
   return 0;
 }
 // weightFactorJacobi_ (32).

@@ -110,3 +110,31 @@ def test_oracle_accepts_generated_model():
     o = Oracle(g["voxels"], (g["fix_nodes"], g["fix_state"]), (g["load_nodes"], g["load_vec"]))
     F = o.get("F").reshape(-1, 3)
     assert np.allclose(F.sum(axis=0), g["direction"], atol=1e-12)
+
+
+def test_generator_supports_on_the_surface():
+    """Every fixed node touches a void voxel or the box boundary: the
+    reference applies fixations only at boundary nodes (1369)."""
+    g = generate(4, (24, 48, 24))
+    vox = g["voxels"]
+    ny, nx, nz = vox.shape
+    pad = np.pad(vox, 1, constant_values=False)
+    ids = np.asarray(g["fix_nodes"])
+    jy, jx, jz = ids % (ny + 1), (ids // (ny + 1)) % (nx + 1), ids // ((ny + 1) * (nx + 1))
+    around = np.zeros(len(ids), int)
+    for dy in (0, 1):
+        for dx in (0, 1):
+            for dz in (0, 1):
+                around += pad[jy + dy, jx + dx, jz + dz]
+    assert len(ids) > 0 and np.all((around > 0) & (around < 8))
+
+
+@pytest.mark.usefixtures("oracle_lib")
+def test_oracle_refuses_interior_fixation():
+    from oracle import Oracle
+    vox = np.ones((6, 12, 6), bool)
+    uy, ux = 7, 13
+    fix = np.array([3 + uy * (6 + ux * 3)])  # the centre node: 8 solid voxels around it
+    load = np.array([0 + uy * (12 + ux * 0)])
+    with pytest.raises(ValueError, match="inside the domain"):
+        Oracle(vox, (fix, np.ones((1, 3), np.uint8)), (load, np.array([[0.0, 0.0, -1.0]])))

@@ -17,8 +17,11 @@ TOP3D_XL.m:448-528):
 * the domain is the largest 6-connected component of U intersected with S
   (face connectivity, so every kept voxel shares a face with another and
   the FE mesh is connected through faces).
-* fixations: every node of a solid voxel in the lowest 5% of the domain's z
-  extent, all three DOFs;
+* fixations: the surface nodes (nodes of a solid voxel that also touch a
+  void voxel or the box boundary) in the lowest 5% of the domain's z extent,
+  all three DOFs.  Supports sit on the surface: the reference applies a
+  fixation only at boundary nodes (Solving_SetupKeWithFixedDOFs intersects
+  fixingCond_ with nodesOnBoundary, TOP3D_XL.m:1369);
 * load: a total force of magnitude 1, direction drawn from the seed, split
   evenly over the surface nodes (nodes of a solid voxel that also touch a
   void voxel or the box boundary) in the top 5% of the domain's z extent.
@@ -111,7 +114,7 @@ def generate(seed=0, shape=DEFAULT_SHAPE, n_ellipsoids=40, axes=(20.0, 120.0), s
                 allsolid &= pad[dy:dy + ny + 1, dx:dx + nx + 1, dz:dz + nz + 1]
     surface = nodes & ~allsolid
     jz = np.arange(nz + 1)
-    fixed = nodes & (jz <= z0 + nslab)[None, None, :]
+    fixed = surface & (jz <= z0 + nslab)[None, None, :]
     loaded = surface & (jz >= z1 - nslab)[None, None, :] & ~fixed
     fy, fx, fz = np.nonzero(fixed)
     fix_nodes = _node_id(fy, fx, fz, ny, nx)
