@@ -30,10 +30,12 @@ pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("oracle_lib")]
 
 
 def _parity(vox, fix, load, extra, mode, V0, nLoop, rHat, n_strict, label, tol=1e-3, max_it=800, c_bar=1e-8,
-            x_bar=1e-6):
+            x_bar=1e-6, layers=None):
     import top3d_xl_amd as t3x
     print(f"\n[gpu-trajectory] {label} {mode} {nLoop} iterations, tol {tol}", flush=True)
     g = t3x.Problem(vox, fix, load, extra_loads=extra, tol=tol, maxIT=max_it)
+    if layers:
+        g.set_passive_layers(*layers)
     rg = g.run(mode, V0, nLoop, math.sqrt(3), rHat, nSnap=nLoop)
     g.close()
     ros = []
@@ -41,6 +43,8 @@ def _parity(vox, fix, load, extra, mode, V0, nLoop, rHat, n_strict, label, tol=1
         o = Oracle(vox, fix, load, extra_loads=extra)
         o.set_tol(tol, max_it)
         o.set_order_variant(v)
+        if layers:
+            o.set_passive_layers(*layers)
         ros.append(o.run(mode, V0, nLoop, math.sqrt(3), rHat, nSnap=nLoop))
     ro = ros[0]
     k = min(rg["loops"], ro["loops"])
@@ -77,3 +81,14 @@ def test_config1_two_load_cases_parity():
     load_b = (lb, np.tile([0.0, 1.0, 0.0], (len(lb), 1)) / len(lb))
     _parity(vox, (fix, np.ones((len(fix), 3), np.uint8)), load_a, [load_b], "GLOBAL", 0.12, 6, 0.0, 4,
             "50x100x50 two load cases")
+
+
+def test_irregular_local_passive_parity():
+    """LOCAL mode (Heaviside, p-norm local volume, MMA) on the generated
+    irregular domain with one boundary layer of passive elements (specify
+    PassiveRegions_ = [1 0 0], 37), solved to tol 1e-8."""
+    from top3d_xl_amd.irregular import generate
+    gm = generate(7, (24, 48, 24))
+    _parity(gm["voxels"], (gm["fix_nodes"], gm["fix_state"]), (gm["load_nodes"], gm["load_vec"]), (),
+            "LOCAL", 0.5, 5, 4.0, 5, "irregular seed 7, 24x48x24, passive boundary layer", tol=1e-8,
+            max_it=5000, layers=(1, 0, 0))
