@@ -105,6 +105,8 @@ int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nF
                      const uint8_t* fixState, int nCases, const int* nPerCase, const int* loadNodes,
                      const double* loadVec, const t3x_params* params, t3x_problem** out);
 int t3x_num_load_cases(const t3x_problem* h);
+/* On failure, *out still receives a handle (unless out is NULL) so that
+ * t3x_last_error() can report why; the caller destroys it either way. */
 void t3x_destroy(t3x_problem* h);
 const char* t3x_last_error(const t3x_problem* h);
 /* Passive elements (passiveElements_, 472, 524-527): voxel ids (0-based,
