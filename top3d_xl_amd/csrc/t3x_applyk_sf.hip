@@ -1,0 +1,352 @@
+// Fine-level K*u with the hex8 element operator in sum-factorised form
+// (Solving_KbyU_MatrixFree, TOP3D_XL.m:641-726, with the element matrix of
+// FEA_VoxelBasedElementStiffnessMatrix, 1254-1292).
+//
+// For the unit brick with trilinear shape functions, every block of Ke is a
+// sum of tensor products of three 1-D 2x2 matrices, one per axis:
+//   K1 = d d^T, M1 = h h^T + d d^T / 12, C1 = d h^T (and C1^T),
+//   d = (-1, 1), h = (1/2, 1/2),
+// with Ke(a i, b j) = lam T_ij + mu T_ji + mu delta_ij sum_k T_kk, where T_ij
+// takes the derivative of the test function along i and of the trial
+// function along j.  In the basis of sums and differences along each axis
+// every 1-D factor maps one coefficient to at most one coefficient, so
+//   Ke u_e = S^T W(lam, mu) S u_e,
+// S: sums and differences of each component's 8 nodal values along x, y and
+// z (24 adds per component), W: 21 coupled coefficients with 8 constants.
+// About 190 flops per cell against 1152 for the dense 24x24 product; the
+// operator is the same up to rounding (the constants follow from E, nu and a
+// unit cell, the same inputs as the dense Ke).
+//
+// Tiling and data flow are those of the matrix-core kernel: a work-group
+// owns 15x15 (y, x) node columns and the 16x16 cells around them and marches
+// a chunk of node layers in z (two node layers and the cell moduli
+// double-buffered in LDS, the next layer's global loads in flight during the
+// cell phase); one thread per cell writes E_e Ke u_e to LDS, then each node
+// thread adds the rows of its node from the 4 cells around its column in a
+// fixed order (deterministic, no atomics).  Boundary conditions: fixed DOFs
+// are zeroed when staged (P_free u) and fixed rows return u (703-715,
+// 1338-1437).
+#include <hip/hip_runtime.h>
+
+#include "t3x_kernels.h"
+#include "t3x_reduce.h"
+
+namespace t3x {
+
+namespace {
+
+constexpr int SF_NT = 15;          // node columns per tile edge
+constexpr int SF_CT = SF_NT + 1;   // cells per tile edge
+constexpr int SF_HT = SF_NT + 2;   // staged nodes per tile edge
+constexpr int SF_ZC = 32;          // node layers per work-group (doubled for huge grids)
+constexpr int SF_T = SF_CT * SF_CT;  // 256 threads: one per cell
+
+// Position bits of local node a in the transform: x = dx, y = 1 - dy (the
+// reference's y axis points up the rows), z = dz.
+__device__ __forceinline__ constexpr int sf_pos(int a) {
+  return kLocDx[a] + 2 * (1 - kLocDy[a]) + 4 * kLocDz[a];
+}
+
+// In place: f[p] over positions p = x + 2y + 4z -> coefficients over
+// (sum, difference) per axis, same bit layout.
+template <typename T>
+__device__ __forceinline__ void sf_analyse(T f[8]) {
+#pragma unroll
+  for (int b = 1; b < 8; b <<= 1)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (!(j & b)) {
+        const T s = f[j | b] + f[j], d = f[j | b] - f[j];
+        f[j] = s;
+        f[j | b] = d;
+      }
+}
+
+// Transpose of sf_analyse's differences: coefficients -> nodal values.
+template <typename T>
+__device__ __forceinline__ void sf_synthesise(T f[8]) {
+#pragma unroll
+  for (int b = 4; b >= 1; b >>= 1)
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (!(j & b)) {
+        const T lo = f[j] - f[j | b], hi = f[j] + f[j | b];
+        f[j] = lo;
+        f[j | b] = hi;
+      }
+}
+
+// E * Ke * u for one cell, all three components, in transform space.
+// c[0..7]: the constants for E = 1 (sf_coefficients), scaled here by e.
+template <typename T>
+__device__ __forceinline__ void sf_cell(T X[8], T Y[8], T Z[8], const T* c, T e) {
+  sf_analyse(X);
+  sf_analyse(Y);
+  sf_analyse(Z);
+  T k[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) k[i] = c[i] * e;
+  T ox[8], oy[8], oz[8];
+  const T t = X[1] + Y[2] + Z[4];
+  ox[0] = oy[0] = oz[0] = T(0);
+  ox[1] = k[0] * X[1] + k[1] * t;
+  oy[2] = k[0] * Y[2] + k[1] * t;
+  oz[4] = k[0] * Z[4] + k[1] * t;
+  ox[2] = oy[1] = k[2] * (X[2] + Y[1]);
+  ox[4] = oz[1] = k[2] * (X[4] + Z[1]);
+  oy[4] = oz[2] = k[2] * (Y[4] + Z[2]);
+  ox[3] = k[3] * X[3] + k[4] * Z[6];
+  ox[5] = k[3] * X[5] + k[4] * Y[6];
+  oy[3] = k[3] * Y[3] + k[4] * Z[5];
+  oy[6] = k[3] * Y[6] + k[4] * X[5];
+  oz[5] = k[3] * Z[5] + k[4] * Y[3];
+  oz[6] = k[3] * Z[6] + k[4] * X[3];
+  ox[6] = k[5] * X[6] + k[6] * (Y[5] + Z[3]);
+  oy[5] = k[5] * Y[5] + k[6] * (X[6] + Z[3]);
+  oz[3] = k[5] * Z[3] + k[6] * (X[6] + Y[5]);
+  ox[7] = k[7] * X[7];
+  oy[7] = k[7] * Y[7];
+  oz[7] = k[7] * Z[7];
+  sf_synthesise(ox);
+  sf_synthesise(oy);
+  sf_synthesise(oz);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    X[i] = ox[i];
+    Y[i] = oy[i];
+    Z[i] = oz[i];
+  }
+}
+
+}  // namespace
+
+SfCoef sf_coefficients(double nu) {
+  const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1.0 / (2 * (1 + nu));
+  const double c2 = mu / 6 + (lam + mu) / 12, c3 = mu / 48 + (lam + mu) / 144;
+  return SfCoef{{mu / 8, lam / 16, mu / 16, c2 / 4, lam / 48, mu / 24, mu / 48, c3}};
+}
+
+enum { SEPI_KU = 0, SEPI_RESID = 1, SEPI_SMOOTH = 2 };
+
+// Epilogues per node (as the dense kernels):
+//   SEPI_KU     y = Ku                         partials: u.y (the PCG's p.q)
+//   SEPI_RESID  y = r - Ku                     (the 'Standard' cycle's defect, 761)
+//   SEPI_SMOOTH y = u + omega (r - Ku) / d     partials: y.r (post-smoothing, 775)
+// T: arithmetic/LDS type; TE, TU, TY, TR, TD: types of E, u, y, r, diag in HBM.
+template <typename T, typename TE, typename TU, typename TY, int EPI, typename TR, typename TD>
+__global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, const TE* __restrict__ E,
+                                                     const uint8_t* __restrict__ fix, const TU* __restrict__ u,
+                                                     TY* __restrict__ y, double* __restrict__ partials,
+                                                     const int* fl, int reduce, const TR* __restrict__ rv,
+                                                     const TD* __restrict__ dv, double omega) {
+  if (fl && fl[FL_DONE]) return;
+  constexpr int CS = SF_HT * SF_HT;       // one component of one staged node layer
+  __shared__ T sU[2 * 3 * CS];            // [buffer][component][x][y]
+  __shared__ T sE[2][SF_CT * SF_CT];      // [buffer][cell y + 16 cell x]
+  __shared__ T sY[24][SF_CT * SF_CT];     // [row][cell y + 16 cell x]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int y0 = blockIdx.x * SF_NT, x0 = blockIdx.y * SF_NT;
+  const int zc0 = blockIdx.z * zc;
+  const int zc1 = min(g.NZ, zc0 + zc);
+  const long long NN = g.NN;
+  const long long plane = (long long)g.NY * g.NX;
+  T cst[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) cst[i] = T(cf.k[i]);
+  // staging: global loads a layer ahead into registers, LDS stores after the
+  // cell phase has released the buffer
+  constexpr int NST = (CS + SF_T - 1) / SF_T;  // staged nodes per thread (2)
+  T pu[NST][3], pe;
+  auto load_stage = [&](int z) {
+#pragma unroll
+    for (int s2 = 0; s2 < NST; ++s2) {
+      const int t = tid + s2 * SF_T;
+      const int hy = t % SF_HT, hx = t / SF_HT;
+      const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+      pu[s2][0] = pu[s2][1] = pu[s2][2] = T(0);
+      if (t < CS && z >= 0 && z < g.NZ && gy >= 0 && gx >= 0 && gy < g.NY && gx < g.NX) {
+        const long long n = gy + (long long)g.NY * gx + plane * z;
+        const uint8_t f = fix[n];
+        pu[s2][0] = (f & 1) ? T(0) : T(u[n]);
+        pu[s2][1] = (f & 2) ? T(0) : T(u[NN + n]);
+        pu[s2][2] = (f & 4) ? T(0) : T(u[2 * NN + n]);
+      }
+    }
+    {
+      const int ey = tid % SF_CT, ex = tid / SF_CT;
+      const int gy = y0 - 1 + ey, gx = x0 - 1 + ex;
+      pe = T(0);
+      if (z >= 0 && z < g.nz && gy >= 0 && gx >= 0 && gy < g.ny && gx < g.nx)
+        pe = T(E[gy + (long long)g.NY * gx + plane * z]);
+    }
+  };
+  auto store_stage = [&](int b) {
+#pragma unroll
+    for (int s2 = 0; s2 < NST; ++s2) {
+      const int t = tid + s2 * SF_T;
+      if (t < CS) {
+        const int hy = t % SF_HT, hx = t / SF_HT;
+        sU[(b * 3 + 0) * CS + hx * SF_HT + hy] = pu[s2][0];
+        sU[(b * 3 + 1) * CS + hx * SF_HT + hy] = pu[s2][1];
+        sU[(b * 3 + 2) * CS + hx * SF_HT + hy] = pu[s2][2];
+      }
+    }
+    sE[b][tid] = pe;
+  };
+  // cell thread: (cy, cx) in 16 x 16; node thread of the gather: (ty, tx) in 15 x 15
+  const int cy = tid % SF_CT, cx = tid / SF_CT;
+  const int ty = cy, tx = cx;
+  const bool gnode = ty < SF_NT && tx < SF_NT;
+  const int yy = y0 + ty, xx = x0 + tx;
+  const bool own = gnode && yy < g.NY && xx < g.NX;
+  T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+  double pq = 0.0;
+  const int kBeg = zc0 - 1;
+  load_stage(kBeg);
+  store_stage(kBeg & 1);
+  load_stage(kBeg + 1);
+  store_stage((kBeg + 1) & 1);
+  for (int k = kBeg; k < zc1; ++k) {
+    const bool more = k + 2 <= zc1;
+    if (more) load_stage(k + 2);  // in flight during this layer's cell phase
+    const bool epi = k >= zc0 && own;
+    const long long n = yy + (long long)g.NY * xx + plane * k;
+    uint8_t f = 0;
+    double uo[3] = {0.0, 0.0, 0.0}, ro[3] = {0.0, 0.0, 0.0};
+    T dd[3] = {T(0), T(0), T(0)};
+    if (epi) {
+      f = fix[n];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        uo[c] = u[c * NN + n];
+        if (EPI != SEPI_KU) ro[c] = double(rv[c * NN + n]);
+        if (EPI == SEPI_SMOOTH) dd[c] = T(dv[c * NN + n]);
+      }
+    }
+    __syncthreads();  // node layers k, k+1 and cell layer k's moduli staged; sY free
+    const int bk = k & 1, bk1 = (k + 1) & 1;
+    {  // ---- cell phase: cell (cy, cx) of layer k
+      T X[8], Yv[8], Z[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int p = sf_pos(a);
+        const int o = ((kLocDz[a] ? bk1 : bk) * 3) * CS + (cx + kLocDx[a]) * SF_HT + (cy + kLocDy[a]);
+        X[p] = sU[o];
+        Yv[p] = sU[o + CS];
+        Z[p] = sU[o + 2 * CS];
+      }
+      sf_cell(X, Yv, Z, cst, sE[bk][tid]);
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int p = sf_pos(a);
+        sY[3 * a + 0][tid] = X[p];
+        sY[3 * a + 1][tid] = Yv[p];
+        sY[3 * a + 2][tid] = Z[p];
+      }
+    }
+    __syncthreads();  // sY complete; node layer k's buffer free
+    if (more) store_stage(bk);
+    // ---- gather: rows of this node (bottom: layer k, top: layer k+1) from its 4 cells
+    if (gnode) {
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc) {
+        const int CY = cc & 1, CX = cc >> 1;
+        const int col = (ty + CY) + SF_CT * (tx + CX);
+        const int jb = loc_index(1 - CY, 1 - CX, 0), jt = loc_index(1 - CY, 1 - CX, 1);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          a0[r] += sY[3 * jb + r][col];
+          a1[r] += sY[3 * jt + r][col];
+        }
+      }
+    }
+    if (epi) {  // node layer k complete
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T ku = ((f >> c) & 1) ? T(uo[c]) : a0[c];
+        if (EPI == SEPI_KU) {
+          const TY v = TY(ku);
+          y[c * NN + n] = v;
+          pq += uo[c] * double(v);
+        } else if (EPI == SEPI_RESID) {
+          y[c * NN + n] = TY(T(ro[c]) - ku);
+        } else {
+          const TY v = dd[c] != T(0) ? TY(T(uo[c]) + T(omega) * (T(ro[c]) - ku) / dd[c]) : TY(0);
+          y[c * NN + n] = v;
+          pq += double(v) * ro[c];
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      a0[r] = a1[r];
+      a1[r] = 0;
+    }
+  }
+  if (reduce) {
+    __shared__ double shp[SF_T / 64];
+    const double v = wave_sum(pq);
+    if (lane == 0) shp[wave] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < SF_T / 64; ++w) s += shp[w];
+      partials[blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z)] = s;
+    }
+  }
+}
+
+static int sf_zc(const Geo& g) {
+  const long long bxy = (long long)((g.NY + SF_NT - 1) / SF_NT) * ((g.NX + SF_NT - 1) / SF_NT);
+  int zc = SF_ZC;
+  while (bxy * ((g.NZ + zc - 1) / zc) > kMaxPartials) zc *= 2;
+  return zc;
+}
+
+static dim3 sf_grid(const Geo& g) {
+  const int zc = sf_zc(g);
+  return dim3((g.NY + SF_NT - 1) / SF_NT, (g.NX + SF_NT - 1) / SF_NT, (g.NZ + zc - 1) / zc);
+}
+
+int applyk_sf_blocks(const Geo& g) {
+  const dim3 d = sf_grid(g);
+  return (int)(d.x * d.y * d.z);
+}
+
+void launch_apply_K_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
+                       const double* u, double* y, double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_KU, double, double>), sf_grid(g), dim3(SF_T),
+                     0, s, g, sf_zc(g), cf, E, fix, u, y, partials, fl, partials ? 1 : 0, (const double*)nullptr,
+                     (const double*)nullptr, 0.0);
+}
+
+void launch_fine_resid64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
+                            const double* x, const double* r, double* d, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_RESID, double, double>), sf_grid(g),
+                     dim3(SF_T), 0, s, g, sf_zc(g), cf, E, fix, x, d, (double*)nullptr, fl, 0, r,
+                     (const double*)nullptr, 0.0);
+}
+
+void launch_fine_smooth64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
+                             const double* x, const double* r, const double* diag, double omega, double* out,
+                             double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_SMOOTH, double, double>), sf_grid(g),
+                     dim3(SF_T), 0, s, g, sf_zc(g), cf, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag,
+                     omega);
+}
+
+void launch_fine_resid32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const float* E, const uint8_t* fix,
+                            const float* x, const double* r, float* d, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_sf<float, float, float, float, SEPI_RESID, double, float>), sf_grid(g), dim3(SF_T),
+                     0, s, g, sf_zc(g), cf, E, fix, x, d, (double*)nullptr, fl, 0, r, (const float*)nullptr, 0.0);
+}
+
+void launch_fine_smooth32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const float* E, const uint8_t* fix,
+                             const float* x, const double* r, const float* diag, double omega, float* out,
+                             double* partials, const int* fl) {
+  hipLaunchKernelGGL((k_apply_K_sf<float, float, float, float, SEPI_SMOOTH, double, float>), sf_grid(g),
+                     dim3(SF_T), 0, s, g, sf_zc(g), cf, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag,
+                     omega);
+}
+
+}  // namespace t3x

@@ -165,6 +165,7 @@ struct t3x_problem {
   int warnings = 0;
   // finest-level device state
   double* dKe = nullptr;
+  SfCoef sf{};  // Ke of the unit brick as the constants of the sum-factorised K*u
   double* dT2 = nullptr;
   uint8_t* fix = nullptr;
   uint8_t* passive = nullptr;  // passive cells (passiveElements_), or null
@@ -180,9 +181,9 @@ struct t3x_problem {
   double* ceTmp = nullptr;  // one case's element compliance (nCases > 1)
   double *E = nullptr, *x = nullptr, *xPhys = nullptr, *ce = nullptr, *dc = nullptr;
   // mixed precision (precision == 1): FP32 V-cycle output; the 'Standard'
-  // cycle adds FP32 Ke/E and the fine-level smoother vectors
+  // cycle adds FP32 E and the fine-level smoother vectors
   float* z32 = nullptr;
-  float *Ke32 = nullptr, *E32 = nullptr, *x32 = nullptr, *d32 = nullptr;
+  float *E32 = nullptr, *x32 = nullptr, *d32 = nullptr;
   // FP64 'Standard' cycle: fine-level smoother state and defect
   double *x64 = nullptr, *d64 = nullptr;
   void* rtmp = nullptr;  // restriction intermediate / coarse smoother products (largest use)
@@ -409,7 +410,7 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
   add(8 * NN, 6);                                    // filter vectors
   add(8 * NN, 2);                                    // filter diagonals
   if (mixed) add(12 * NN);                           // z32
-  if (prm.vcycle == 1 && mixed) add(576 * 4), add(4 * NN), add(12 * NN, 2);  // Ke32, E32, x32, d32
+  if (prm.vcycle == 1 && mixed) add(4 * NN), add(12 * NN, 2);                // E32, x32, d32
   if (prm.vcycle == 1 && !mixed) add(24 * NN, 2);                            // x64, d64
   if (mode == 1) add(8 * NN, 13);                    // MMA state
   add(576 * 8);                                      // Ke
@@ -576,15 +577,13 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   const std::vector<double> Ke = gauss_ke(P.P.nu, 1.0);
   P.dKe = P.alloc<double>(576);
   P.upload(P.dKe, Ke.data(), 576 * sizeof(double));
+  P.sf = sf_coefficients(P.P.nu);
   if (P.mixed()) P.z32 = P.alloc<float>(3 * NN);
   if (P.standard() && !P.mixed()) {
     P.x64 = P.alloc<double>(3 * NN);
     P.d64 = P.alloc<double>(3 * NN);
   }
   if (P.standard() && P.mixed()) {
-    std::vector<float> Kf(Ke.begin(), Ke.end());
-    P.Ke32 = P.alloc<float>(576);
-    P.upload(P.Ke32, Kf.data(), 576 * sizeof(float));
     P.E32 = P.alloc<float>(NN);
     P.x32 = P.alloc<float>(3 * NN);
     P.d32 = P.alloc<float>(3 * NN);
@@ -796,7 +795,7 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
   if (!x0Ready) launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl);        // 757
-  launch_fine_resid32_mfma(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.d32, fl);              // 761
+  launch_fine_resid32_sf(s, g0, P.sf, P.E32, P.fix, P.x32, r, P.d32, fl);                  // 761
   launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const float*)P.d32, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
@@ -816,9 +815,9 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
     std::swap(V.xf, V.tf);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, P.lv[1].xf, P.x32, P.lv[0].nmask, fl);
-  launch_fine_smooth32_mfma(s, g0, P.Ke32, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z,
-                            fuse ? P.ws.partials : nullptr, fl);
-  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_mfma_blocks(g0), P.ws, sc, (int*)fl, nullptr);
+  launch_fine_smooth32_sf(s, g0, P.sf, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z, fuse ? P.ws.partials : nullptr,
+                          fl);
+  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_sf_blocks(g0), P.ws, sc, (int*)fl, nullptr);
   P.tend(TG_VCYCLE, ev);
 }
 
@@ -833,7 +832,7 @@ void vcycle_standard64(t3x_problem& P, const double* r, double* z, double* sc, c
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
   launch_jacobi0(s, r, (const double*)P.lv[0].diag, P.x64, om, 3 * P.lv[0].NN, fl);          // 757
-  launch_fine_resid64(s, g0, P.dKe, P.E, P.fix, P.x64, r, P.d64, fl);                         // 761
+  launch_fine_resid64_sf(s, g0, P.sf, P.E, P.fix, P.x64, r, P.d64, fl);                      // 761
   launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.d64, tmp, P.lv[1].nmask, P.lv[1].r, fl);
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
@@ -854,8 +853,8 @@ void vcycle_standard64(t3x_problem& P, const double* r, double* z, double* sc, c
     std::swap(V.x, V.t);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.lv[1].x, P.x64, P.lv[0].nmask, fl);
-  launch_fine_smooth64(s, g0, P.dKe, P.E, P.fix, P.x64, r, P.lv[0].diag, om, z, fuse ? P.ws.partials : nullptr, fl);
-  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_tiled_blocks(g0), P.ws, sc, (int*)fl, nullptr);
+  launch_fine_smooth64_sf(s, g0, P.sf, P.E, P.fix, P.x64, r, P.lv[0].diag, om, z, fuse ? P.ws.partials : nullptr, fl);
+  if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_sf_blocks(g0), P.ws, sc, (int*)fl, nullptr);
   P.tend(TG_VCYCLE, ev);
 }
 
@@ -870,7 +869,7 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
   const bool mx = P.mixed();
   launch_pcg_init(s, sc, fl, tol);
   launch_norm2(s, b, n3, P.ws, &sc[SC_NORMB]);                                              // 607
-  launch_apply_K(s, g0, P.dKe, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);               // 609
+  launch_apply_K(s, g0, P.sf, P.E, P.fix, y, P.q, P.ws, sc, nullptr, false);                // 609
   launch_sub(s, P.r, b, P.q, n3);
   if (mx) {
     if (P.standard()) vcycle_standard(P, P.r, P.z32, sc, nullptr, true);                   // 610, 612
@@ -889,7 +888,7 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
   auto iteration = [&]() {
     hipEvent_t ek[2];
     P.tbegin(TG_APPLYK, ek);
-    launch_apply_K(s, g0, P.dKe, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                 // 614-615
+    launch_apply_K(s, g0, P.sf, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                  // 614-615
     P.tend(TG_APPLYK, ek);
     if (mx && P.standard())  // 617-622, with the first fine Jacobi step of 628-629 fused
       launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl, P.lv[0].diagf, P.x32, P.P.omega);
@@ -1690,7 +1689,7 @@ int t3x_assemble(t3x_problem* h) { return guarded(h, [&] { return assemble(*h); 
 int t3x_apply_K(t3x_problem* h, const double* u, double* y) {
   return guarded(h, [&] {
     upload_nodal(*h, u, h->p);
-    launch_apply_K(h->stream, geo(h->lv[0]), h->dKe, h->E, h->fix, h->p, h->q, h->ws, h->sc, nullptr, false);
+    launch_apply_K(h->stream, geo(h->lv[0]), h->sf, h->E, h->fix, h->p, h->q, h->ws, h->sc, nullptr, false);
     download_nodal(*h, h->q, y);
     return (int)T3X_OK;
   });

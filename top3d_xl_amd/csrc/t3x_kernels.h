@@ -32,27 +32,27 @@ enum FinOp { FIN_PQ = 0, FIN_ZR_INIT, FIN_ZR_LOOP, FIN_RR, FIN_SUM, FIN_SQRT };
 void launch_fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out);
 
 // ---------------------------------------------------------------- FEM
-// matrix-core FP32 K*u of the 'Standard' smoother (t3x_applyk_mfma.hip); partials: one per work-group
-int applyk_mfma_blocks(const Geo& g);
-void launch_fine_resid32_mfma(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                              const float* x, const double* r, float* d, const int* fl);
-void launch_fine_smooth32_mfma(hipStream_t s, const Geo& g, const float* Ke, const float* E, const uint8_t* fix,
-                               const float* x, const double* r, const float* diag, double omega, float* out,
-                               double* partials, const int* fl);
-// tiled K*u (t3x_applyk.hip); partials: one per work-group (nullptr: none)
-int applyk_tiled_blocks(const Geo& g);
-void launch_apply_K_tiled(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                          const double* u, double* y, double* partials, const int* fl);
-// FP64 'Standard' smoother products: d = r - Kx; out = x + omega (r - Kx) / diag
-// (partials: out.r per work-group, nullptr: none)
-void launch_fine_resid64(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                         const double* x, const double* r, double* d, const int* fl);
-void launch_fine_smooth64(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                          const double* x, const double* r, const double* diag, double omega, double* out,
-                          double* partials, const int* fl);
-void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                           const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
-void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+// sum-factorised K*u (t3x_applyk_sf.hip): the isotropic unit-brick Ke as
+// 8 constants; partials: one per work-group (nullptr: none)
+struct SfCoef {
+  double k[8];
+};
+SfCoef sf_coefficients(double nu);
+int applyk_sf_blocks(const Geo& g);
+void launch_apply_K_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
+                       const double* u, double* y, double* partials, const int* fl);
+void launch_fine_resid64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
+                            const double* x, const double* r, double* d, const int* fl);
+void launch_fine_smooth64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
+                             const double* x, const double* r, const double* diag, double omega, double* out,
+                             double* partials, const int* fl);
+void launch_fine_resid32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const float* E, const uint8_t* fix,
+                            const float* x, const double* r, float* d, const int* fl);
+void launch_fine_smooth32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const float* E, const uint8_t* fix,
+                             const float* x, const double* r, const float* diag, double omega, float* out,
+                             double* partials, const int* fl);
+// fine-level K*u (y = Ku, with the PCG's p.q reduction when pcgStep)
+void launch_apply_K(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                     const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
 void launch_compliance(hipStream_t s, const Geo& g, const double* Ke, const uint8_t* emask, const double* u,
                        double* ce);

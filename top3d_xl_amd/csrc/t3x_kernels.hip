@@ -87,82 +87,17 @@ __global__ __launch_bounds__(kBlock) void k_fin_max(int nb, const double* __rest
 }
 
 // =========================================================================
-// K*u on the finest level (Solving_KbyU_MatrixFree, 641-726).
-//
-// Node-centric gather: y_n = sum over the <= 8 cells around n of
-// E_e * Ke[rows of n] * u_e, with the fixed DOFs handled as
+// K*u on the finest level (Solving_KbyU_MatrixFree, 641-726), through the
+// sum-factorised kernel of t3x_applyk_sf.hip.  The fixed DOFs are handled as
 // P_free K(E) P_free + I_fixed — identical to the per-element
-// uniqueKesFree_*E + uniqueKesFixed_ construction (1338-1437, 703-715):
-// the fixed rows/cols of every touching element are zero and the 1/nAdj
-// diagonal entries of the touching elements add up to 1.
+// uniqueKesFree_*E + uniqueKesFixed_ construction (1338-1437, 703-715): the
+// fixed rows/cols of every touching element are zero and the 1/nAdj diagonal
+// entries of the touching elements add up to 1.
 // =========================================================================
-__global__ __launch_bounds__(kBlock) void k_apply_K(Geo g, const double* __restrict__ Ke,
-                                                    const double* __restrict__ E,
-                                                    const uint8_t* __restrict__ fix,
-                                                    const double* __restrict__ u, double* __restrict__ y,
-                                                    RedWs ws, double* sc, const int* fl, int pcgStep) {
-  if (fl && fl[FL_DONE]) return;
-  const long long NN = g.NN;
-  double pq = 0.0;
-  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < NN; n += (long long)gridDim.x * kBlock) {
-    int iy, ix, iz;
-    node_yxz(n, g, iy, ix, iz);
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int dy = c & 1, dx = (c >> 1) & 1, dz = c >> 2;  // offset of n inside the cell
-      const int cy = iy - dy, cx = ix - dx, cz = iz - dz;
-      if (!cell_in(g, cy, cx, cz)) continue;
-      const long long ce = nidx(g, cy, cx, cz);
-      const double Ee = E[ce];
-      if (Ee == 0.0) continue;
-      const int jn = loc_index(dy, dx, dz);
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const long long nb = nidx(g, cy + kLocDy[j], cx + kLocDx[j], cz + kLocDz[j]);
-        const uint8_t f = fix[nb];
-        const double v0 = (f & 1) ? 0.0 : u[nb];
-        const double v1 = (f & 2) ? 0.0 : u[NN + nb];
-        const double v2 = (f & 4) ? 0.0 : u[2 * NN + nb];
-        s0 += Ke[(3 * jn + 0) * 24 + 3 * j + 0] * v0 + Ke[(3 * jn + 0) * 24 + 3 * j + 1] * v1 +
-              Ke[(3 * jn + 0) * 24 + 3 * j + 2] * v2;
-        s1 += Ke[(3 * jn + 1) * 24 + 3 * j + 0] * v0 + Ke[(3 * jn + 1) * 24 + 3 * j + 1] * v1 +
-              Ke[(3 * jn + 1) * 24 + 3 * j + 2] * v2;
-        s2 += Ke[(3 * jn + 2) * 24 + 3 * j + 0] * v0 + Ke[(3 * jn + 2) * 24 + 3 * j + 1] * v1 +
-              Ke[(3 * jn + 2) * 24 + 3 * j + 2] * v2;
-      }
-      a0 += s0 * Ee;
-      a1 += s1 * Ee;
-      a2 += s2 * Ee;
-    }
-    const uint8_t f = fix[n];
-    const double u0 = u[n], u1 = u[NN + n], u2 = u[2 * NN + n];
-    const double y0 = (f & 1) ? u0 : a0, y1 = (f & 2) ? u1 : a1, y2 = (f & 4) ? u2 : a2;
-    y[n] = y0;
-    y[NN + n] = y1;
-    y[2 * NN + n] = y2;
-    pq += u0 * y0 + u1 * y1 + u2 * y2;
-  }
-  if (pcgStep) {
-    double v[1] = {pq};
-    write_partials<1>(v, ws.partials);
-  }
-}
-
-void launch_apply_K_simple(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
-                           const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
-  const int nb = pcgStep ? red_blocks(g.NN) : grid_for(g.NN);
-  hipLaunchKernelGGL(k_apply_K, dim3(nb), dim3(kBlock), 0, s, g, Ke, E, fix, u, y, ws, sc, fl, pcgStep ? 1 : 0);
-  if (pcgStep) fin1(s, FIN_PQ, nb, ws, sc, (int*)fl, nullptr);
-}
-
-void launch_apply_K(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
+void launch_apply_K(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                     const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep) {
-  // the vector kernel: on gfx950 it outruns the FP64 matrix-core variant
-  // (1.78 vs 1.96 ms at 256x512x256), whose 67 KB of LDS caps it at 2 waves/SIMD
-  launch_apply_K_tiled(s, g, Ke, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
-  if (pcgStep) fin1(s, FIN_PQ, applyk_tiled_blocks(g), ws, sc, (int*)fl, nullptr);
+  launch_apply_K_sf(s, g, cf, E, fix, u, y, pcgStep ? ws.partials : nullptr, fl);
+  if (pcgStep) fin1(s, FIN_PQ, applyk_sf_blocks(g), ws, sc, (int*)fl, nullptr);
 }
 
 // =========================================================================
