@@ -17,13 +17,13 @@
 // operator is the same up to rounding (the constants follow from E, nu and a
 // unit cell, the same inputs as the dense Ke).
 //
-// Tiling and data flow are those of the matrix-core kernel: a work-group
+// Tiling: a work-group
 // owns 15x15 (y, x) node columns and the 16x16 cells around them and marches
 // a chunk of node layers in z (two node layers and the cell moduli
 // double-buffered in LDS, the next layer's global loads in flight during the
-// cell phase); one thread per cell writes E_e Ke u_e to LDS, then each node
-// thread adds the rows of its node from the 4 cells around its column in a
-// fixed order (deterministic, no atomics).  Boundary conditions: fixed DOFs
+// cell phase); one thread per cell forms E_e Ke u_e, then each node thread
+// adds the rows of its node from the 4 cells around its column in a fixed
+// order (deterministic, no atomics).  Boundary conditions: fixed DOFs
 // are zeroed when staged (P_free u) and fixed rows return u (703-715,
 // 1338-1437).
 #include <hip/hip_runtime.h>
@@ -74,6 +74,17 @@ __device__ __forceinline__ void sf_synthesise(T f[8]) {
         f[j] = lo;
         f[j | b] = hi;
       }
+}
+
+// Value of lane + 1 within its row of 16 lanes (DPP row_shl:1; the last lane
+// of a row reads 0).  Every lane of the wave must be active.
+__device__ __forceinline__ float next_in_row(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double next_in_row(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x101, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x101, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
 // E * Ke * u for one cell, all three components, in transform space.
@@ -143,7 +154,14 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   constexpr int CS = SF_HT * SF_HT;       // one component of one staged node layer
   __shared__ T sU[2 * 3 * CS];            // [buffer][component][x][y]
   __shared__ T sE[2][SF_CT * SF_CT];      // [buffer][cell y + 16 cell x]
-  __shared__ T sY[24][SF_CT * SF_CT];     // [row][cell y + 16 cell x]
+  // Gather of the cell rows into node rows.  FP64: the y-neighbour pair is
+  // summed in registers across lanes and only the x-neighbour partial sums
+  // (6 rows) go through LDS, which keeps the work-group at 30 KB of LDS
+  // (4 waves per SIMD instead of 2).  FP32: all 24 rows of every cell go
+  // through LDS (measured faster there: 0.89 / 1.04 ms against 0.90 / 1.11 ms
+  // for the residual / smoothing launches at 256x512x256).
+  constexpr bool kRowDpp = sizeof(T) == 8;
+  __shared__ T sR[kRowDpp ? 6 : 24][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int y0 = blockIdx.x * SF_NT, x0 = blockIdx.y * SF_NT;
   const int zc0 = blockIdx.z * zc;
@@ -223,9 +241,16 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
         if (EPI == SEPI_SMOOTH) dd[c] = T(dv[c * NN + n]);
       }
     }
-    __syncthreads();  // node layers k, k+1 and cell layer k's moduli staged; sY free
+    __syncthreads();  // node layers k, k+1 and cell layer k's moduli staged; sR free
     const int bk = k & 1, bk1 = (k + 1) & 1;
-    {  // ---- cell phase: cell (cy, cx) of layer k
+    // ---- cell phase: cell (cy, cx) of layer k.  Lanes hold cells y-fastest in
+    // rows of 16, so the cell at y + 1 is the next lane of the row: the node
+    // thread (ty, tx) = (cy, cx) sums its rows from cells (cy, cx) and
+    // (cy + 1, cx) in registers (q), and the pair at x + 1 comes through LDS
+    // from the thread of column cx + 1, which formed the same sum for the
+    // node at its x - 1 (xr).
+    T q[6];
+    {
       T X[8], Yv[8], Z[8];
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
@@ -236,27 +261,50 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
         Z[p] = sU[o + 2 * CS];
       }
       sf_cell(X, Yv, Z, cst, sE[bk][tid]);
+      if constexpr (kRowDpp) {
+        // transform position of local (dy, dx, dz): dx + 2 (1 - dy) + 4 dz
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        const int p = sf_pos(a);
-        sY[3 * a + 0][tid] = X[p];
-        sY[3 * a + 1][tid] = Yv[p];
-        sY[3 * a + 2][tid] = Z[p];
+        for (int dz = 0; dz < 2; ++dz) {
+          const int p11 = 1 + 4 * dz, p01 = 3 + 4 * dz;  // (dy, dx) = (1, 1), (0, 1): this node column
+          const int p10 = 0 + 4 * dz, p00 = 2 + 4 * dz;  // (1, 0), (0, 0): the column at x - 1
+          q[3 * dz + 0] = X[p11] + next_in_row(X[p01]);
+          q[3 * dz + 1] = Yv[p11] + next_in_row(Yv[p01]);
+          q[3 * dz + 2] = Z[p11] + next_in_row(Z[p01]);
+          sR[3 * dz + 0][tid] = X[p10] + next_in_row(X[p00]);
+          sR[3 * dz + 1][tid] = Yv[p10] + next_in_row(Yv[p00]);
+          sR[3 * dz + 2][tid] = Z[p10] + next_in_row(Z[p00]);
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const int p = sf_pos(a);
+          sR[3 * a + 0][tid] = X[p];
+          sR[3 * a + 1][tid] = Yv[p];
+          sR[3 * a + 2][tid] = Z[p];
+        }
       }
     }
-    __syncthreads();  // sY complete; node layer k's buffer free
+    __syncthreads();  // sR complete; node layer k's buffer free
     if (more) store_stage(bk);
-    // ---- gather: rows of this node (bottom: layer k, top: layer k+1) from its 4 cells
+    // ---- gather: bottom rows (layer k) and top rows (layer k+1) of this node
     if (gnode) {
-#pragma unroll
-      for (int cc = 0; cc < 4; ++cc) {
-        const int CY = cc & 1, CX = cc >> 1;
-        const int col = (ty + CY) + SF_CT * (tx + CX);
-        const int jb = loc_index(1 - CY, 1 - CX, 0), jt = loc_index(1 - CY, 1 - CX, 1);
+      if constexpr (kRowDpp) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-          a0[r] += sY[3 * jb + r][col];
-          a1[r] += sY[3 * jt + r][col];
+          a0[r] += q[r] + sR[r][tid + SF_CT];
+          a1[r] += q[3 + r] + sR[3 + r][tid + SF_CT];
+        }
+      } else {  // cells (cy + CY, cx + CX), the node at local (1 - CY, 1 - CX)
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+          const int CY = cc & 1, CX = cc >> 1;
+          const int col = tid + CY + SF_CT * CX;
+          const int jb = loc_index(1 - CY, 1 - CX, 0), jt = loc_index(1 - CY, 1 - CX, 1);
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {
+            a0[r] += sR[3 * jb + r][col];
+            a1[r] += sR[3 * jt + r][col];
+          }
         }
       }
     }
