@@ -163,8 +163,23 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   constexpr bool kRowDpp = sizeof(T) == 8;
   __shared__ T sR[kRowDpp ? 6 : 24][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int y0 = blockIdx.x * SF_NT, x0 = blockIdx.y * SF_NT;
-  const int zc0 = blockIdx.z * zc;
+  // XCD-aware tile order: work-groups are dispatched round-robin over the 8
+  // XCDs (dispatch slot b on XCD b % 8); XCD x gets the contiguous range of
+  // tiles [x q + min(x, r), ...) in (y, x, z) order, so tiles that share halo
+  // cache lines share an L2 (measured at 256x512x256: FP64 K*p 0.88 -> 0.82
+  // ms, FP32 smoothing step 1.04 -> 0.89 ms)
+  int bx, by, bz;
+  {
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int q = nb / 8, r = nb % 8, x = b % 8;
+    const int t = x * q + min(x, r) + b / 8;
+    bx = t % gridDim.x;
+    by = (t / gridDim.x) % gridDim.y;
+    bz = t / (gridDim.x * gridDim.y);
+  }
+  const int y0 = bx * SF_NT, x0 = by * SF_NT;
+  const int zc0 = bz * zc;
   const int zc1 = min(g.NZ, zc0 + zc);
   const long long NN = g.NN;
   const long long plane = (long long)g.NY * g.NX;
@@ -339,7 +354,7 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < SF_T / 64; ++w) s += shp[w];
-      partials[blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z)] = s;
+      partials[bx + (size_t)gridDim.x * (by + (size_t)gridDim.y * bz)] = s;
     }
   }
 }
