@@ -256,3 +256,30 @@ def test_config2_local_parity(oracle_lib):
     sqrt 3): first 8 iterations, same bars and noise-floor rule as config 1;
     the strict bar must hold at least over the first 3 iterations."""
     _trajectory_parity("LOCAL", 0.5, 8, 6.0, 3)
+
+
+@pytest.mark.parametrize("nu", [0.2, 0.45])
+def test_apply_K_poisson_ratio(nu):
+    """The fine K*u takes the element operator's constants from nu (sum-
+    factorised form): against the scipy-assembled P_free K P_free + I_fixed
+    built from the Gauss-quadrature Ke at the same nu, on the irregular
+    5x9x5 model with partial fixations."""
+    import top3d_xl_amd as t3x
+    from tests import bruteforce as bf
+    from tests.test_oracle_pins import grid_for
+    print(f"\n[gpu-parity] apply_K nu={nu}", flush=True)
+    vox, fix, load = small_irregular()
+    g = t3x.Problem(vox, fix, load, nu=nu)
+    try:
+        grid = grid_for(vox, fix, load)
+        grid.Ke = bf.ref_ke_gauss(nu)
+        rng = np.random.default_rng(5)
+        E = 1e-6 + rng.random(g.nE) ** 3
+        g.set_modulus(E)
+        g.assemble()
+        Kbc = grid.K_bc(E)
+        for _ in range(2):
+            u = rng.standard_normal(3 * g.nN)
+            assert rel(g.apply_K(u), Kbc @ u) < 1e-13
+    finally:
+        g.close()

@@ -13,19 +13,18 @@
 //   Ke u_e = S^T W(lam, mu) S u_e,
 // S: sums and differences of each component's 8 nodal values along x, y and
 // z (24 adds per component), W: 21 coupled coefficients with 8 constants.
-// About 190 flops per cell against 1152 for the dense 24x24 product; the
+// 196 flops per cell against 1152 for the dense 24x24 product; the
 // operator is the same up to rounding (the constants follow from E, nu and a
 // unit cell, the same inputs as the dense Ke).
 //
-// Tiling: a work-group
-// owns 15x15 (y, x) node columns and the 16x16 cells around them and marches
-// a chunk of node layers in z (two node layers and the cell moduli
-// double-buffered in LDS, the next layer's global loads in flight during the
-// cell phase); one thread per cell forms E_e Ke u_e, then each node thread
-// adds the rows of its node from the 4 cells around its column in a fixed
-// order (deterministic, no atomics).  Boundary conditions: fixed DOFs
-// are zeroed when staged (P_free u) and fixed rows return u (703-715,
-// 1338-1437).
+// Tiling: a work-group owns 15x15 (y, x) node columns and the 16x16 cells
+// around them and marches a chunk of node layers in z (two node layers and
+// the cell moduli double-buffered in LDS, the next layer's global loads in
+// flight during the cell phase); one thread per cell forms E_e Ke u_e, then
+// each node thread adds the rows of its node from the 4 cells around its
+// column in a fixed order (deterministic, no atomics).  Boundary conditions:
+// fixed DOFs are zeroed when staged (P_free u) and fixed rows return u
+// (703-715, 1338-1437).
 #include <hip/hip_runtime.h>
 
 #include "t3x_kernels.h"
