@@ -153,14 +153,11 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   constexpr int CS = SF_HT * SF_HT;       // one component of one staged node layer
   __shared__ T sU[2 * 3 * CS];            // [buffer][component][x][y]
   __shared__ T sE[2][SF_CT * SF_CT];      // [buffer][cell y + 16 cell x]
-  // Gather of the cell rows into node rows.  FP64: the y-neighbour pair is
-  // summed in registers across lanes and only the x-neighbour partial sums
-  // (6 rows) go through LDS, which keeps the work-group at 30 KB of LDS
-  // (4 waves per SIMD instead of 2).  FP32: all 24 rows of every cell go
-  // through LDS (measured faster there: 0.89 / 1.04 ms against 0.90 / 1.11 ms
-  // for the residual / smoothing launches at 256x512x256).
-  constexpr bool kRowDpp = sizeof(T) == 8;
-  __shared__ T sR[kRowDpp ? 6 : 24][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
+  // Gather of the cell rows into node rows: the y-neighbour pair is summed in
+  // registers across lanes and only the x-neighbour partial sums (6 rows) go
+  // through LDS; in FP64 that keeps the work-group at 30 KB of LDS (4 waves
+  // per SIMD; 2 with all 24 rows in LDS).
+  __shared__ T sR[6][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware tile order: work-groups are dispatched round-robin over the 8
   // XCDs (dispatch slot b on XCD b % 8); XCD x gets the contiguous range of
@@ -261,8 +258,8 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
     // rows of 16, so the cell at y + 1 is the next lane of the row: the node
     // thread (ty, tx) = (cy, cx) sums its rows from cells (cy, cx) and
     // (cy + 1, cx) in registers (q), and the pair at x + 1 comes through LDS
-    // from the thread of column cx + 1, which formed the same sum for the
-    // node at its x - 1 (xr).
+    // (sR) from the thread of column cx + 1, which formed the same sum for
+    // the node at its x - 1.
     T q[6];
     {
       T X[8], Yv[8], Z[8];
@@ -275,51 +272,27 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
         Z[p] = sU[o + 2 * CS];
       }
       sf_cell(X, Yv, Z, cst, sE[bk][tid]);
-      if constexpr (kRowDpp) {
-        // transform position of local (dy, dx, dz): dx + 2 (1 - dy) + 4 dz
+      // transform position of local (dy, dx, dz): dx + 2 (1 - dy) + 4 dz
 #pragma unroll
-        for (int dz = 0; dz < 2; ++dz) {
-          const int p11 = 1 + 4 * dz, p01 = 3 + 4 * dz;  // (dy, dx) = (1, 1), (0, 1): this node column
-          const int p10 = 0 + 4 * dz, p00 = 2 + 4 * dz;  // (1, 0), (0, 0): the column at x - 1
-          q[3 * dz + 0] = X[p11] + next_in_row(X[p01]);
-          q[3 * dz + 1] = Yv[p11] + next_in_row(Yv[p01]);
-          q[3 * dz + 2] = Z[p11] + next_in_row(Z[p01]);
-          sR[3 * dz + 0][tid] = X[p10] + next_in_row(X[p00]);
-          sR[3 * dz + 1][tid] = Yv[p10] + next_in_row(Yv[p00]);
-          sR[3 * dz + 2][tid] = Z[p10] + next_in_row(Z[p00]);
-        }
-      } else {
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          const int p = sf_pos(a);
-          sR[3 * a + 0][tid] = X[p];
-          sR[3 * a + 1][tid] = Yv[p];
-          sR[3 * a + 2][tid] = Z[p];
-        }
+      for (int dz = 0; dz < 2; ++dz) {
+        const int p11 = 1 + 4 * dz, p01 = 3 + 4 * dz;  // (dy, dx) = (1, 1), (0, 1): this node column
+        const int p10 = 0 + 4 * dz, p00 = 2 + 4 * dz;  // (1, 0), (0, 0): the column at x - 1
+        q[3 * dz + 0] = X[p11] + next_in_row(X[p01]);
+        q[3 * dz + 1] = Yv[p11] + next_in_row(Yv[p01]);
+        q[3 * dz + 2] = Z[p11] + next_in_row(Z[p01]);
+        sR[3 * dz + 0][tid] = X[p10] + next_in_row(X[p00]);
+        sR[3 * dz + 1][tid] = Yv[p10] + next_in_row(Yv[p00]);
+        sR[3 * dz + 2][tid] = Z[p10] + next_in_row(Z[p00]);
       }
     }
     __syncthreads();  // sR complete; node layer k's buffer free
     if (more) store_stage(bk);
     // ---- gather: bottom rows (layer k) and top rows (layer k+1) of this node
     if (gnode) {
-      if constexpr (kRowDpp) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          a0[r] += q[r] + sR[r][tid + SF_CT];
-          a1[r] += q[3 + r] + sR[3 + r][tid + SF_CT];
-        }
-      } else {  // cells (cy + CY, cx + CX), the node at local (1 - CY, 1 - CX)
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-          const int CY = cc & 1, CX = cc >> 1;
-          const int col = tid + CY + SF_CT * CX;
-          const int jb = loc_index(1 - CY, 1 - CX, 0), jt = loc_index(1 - CY, 1 - CX, 1);
-#pragma unroll
-          for (int r = 0; r < 3; ++r) {
-            a0[r] += sR[3 * jb + r][col];
-            a1[r] += sR[3 * jt + r][col];
-          }
-        }
+      for (int r = 0; r < 3; ++r) {
+        a0[r] += q[r] + sR[r][tid + SF_CT];
+        a1[r] += q[3 + r] + sR[3 + r][tid + SF_CT];
       }
     }
     if (epi) {  // node layer k complete
