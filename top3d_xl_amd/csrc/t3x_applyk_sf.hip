@@ -161,21 +161,11 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   // per SIMD; 2 with all 24 rows in LDS).
   __shared__ T sR[6][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // XCD-aware tile order: work-groups are dispatched round-robin over the 8
-  // XCDs (dispatch slot b on XCD b % 8); XCD x gets the contiguous range of
-  // tiles [x q + min(x, r), ...) in (y, x, z) order, so tiles that share halo
-  // cache lines share an L2 (measured at 256x512x256: FP64 K*p 0.88 -> 0.82
-  // ms, FP32 smoothing step 1.04 -> 0.89 ms)
-  int bx, by, bz;
-  {
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    const int q = nb / 8, r = nb % 8, x = b % 8;
-    const int t = x * q + min(x, r) + b / 8;
-    bx = t % gridDim.x;
-    by = (t / gridDim.x) % gridDim.y;
-    bz = t / (gridDim.x * gridDim.y);
-  }
+  // XCD-aware tile order (xcd_tile_order): neighbouring tiles share an L2
+  // (measured at 256x512x256: FP64 K*p 0.88 -> 0.82 ms, FP32 smoothing step
+  // 1.04 -> 0.89 ms)
+  const dim3 tile = xcd_tile_order();
+  const int bx = tile.x, by = tile.y, bz = tile.z;
   const int y0 = bx * SF_NT, x0 = by * SF_NT;
   const int zc0 = bz * zc;
   const int zc1 = min(g.NZ, zc0 + zc);

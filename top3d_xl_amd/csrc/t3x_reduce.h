@@ -16,6 +16,19 @@ constexpr int kBlock = 256;
 constexpr int kMaxRedBlocks = 2048;   // grid-stride reductions: blocks (x up to 8 values each)
 constexpr int kMaxPartials = 65536;   // partials buffer (doubles): one-value reductions over bigger grids
 
+// XCD-aware order of a 3-D grid of tiles.  Work-groups are dispatched
+// round-robin over the 8 XCDs (linear dispatch slot b runs on XCD b % 8), each
+// with its own L2; this maps slot b to tile t so that XCD x gets the
+// contiguous range of tiles [x q + min(x, r), ...) in (x, y, z) order, and
+// tiles that share halo cache lines share an L2.  A bijection on the grid.
+__device__ __forceinline__ dim3 xcd_tile_order() {
+  const int nb = gridDim.x * gridDim.y * gridDim.z;
+  const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int q = nb / 8, r = nb % 8, x = b % 8;
+  const int t = x * q + min(x, r) + b / 8;
+  return dim3(t % gridDim.x, (t / gridDim.x) % gridDim.y, t / (gridDim.x * gridDim.y));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
