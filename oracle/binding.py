@@ -60,6 +60,13 @@ def _load():
     L.orc_oc.restype = C.c_double
     L.orc_oc.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_int)]
     L.orc_mma.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
+    L.orc_mma_ex.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
+    L.orc_heaviside.argtypes = [C.c_int, C.c_double, C.c_double, _dp, C.c_void_p, C.c_void_p]
+    L.orc_pnorm.restype = C.c_double
+    L.orc_pnorm.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.orc_compliance_sens.argtypes = [vp, _dp, _dp, C.c_double, _dp]
+    L.orc_local_sens.restype = C.c_double
+    L.orc_local_sens.argtypes = [vp, C.c_double, _dp, _dp, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
     L.orc_run.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
                           _dp, _dp, _dp, _dp, _dp, _ip, _dp, C.c_void_p, C.c_int,
                           C.POINTER(C.c_double), C.POINTER(C.c_int), _ip, _dp]
@@ -236,6 +243,58 @@ class Oracle:
         xv, xmn, xmx, xo1, xo2, df, dg, lo, up = arrs
         nn = L.orc_mma(n, iter_, xv, xmn, xmx, xo1, xo2, df, float(gx), dg, lo, up)
         return dict(x=xv, xold1=xo1, xold2=xo2, low=lo, upp=up, newton=nn)
+
+    @staticmethod
+    def mma_ex(iter_, xval, xmin, xmax, xold1, xold2, dfdx, gx, dgdx, low, upp):
+        """One MMAseq update that also returns GenSub's subproblem (alpha,
+        beta, p0, q0, pij, qij, b) and SolveDIP's final dual (lam, mu)."""
+        L = _load()
+        n = len(xval)
+        arrs = [np.array(a, dtype=np.float64) for a in (xval, xmin, xmax, xold1, xold2, dfdx, dgdx, low, upp)]
+        xv, xmn, xmx, xo1, xo2, df, dg, lo, up = arrs
+        sub = np.zeros(6 * n + 3)
+        nn = L.orc_mma_ex(n, iter_, xv, xmn, xmx, xo1, xo2, df, float(gx), dg, lo, up, sub)
+        parts = sub[:6 * n].reshape(6, n)
+        return dict(x=xv, xold1=xo1, xold2=xo2, low=lo, upp=up, newton=nn, alpha=parts[0], beta=parts[1],
+                    p0=parts[2], q0=parts[3], pij=parts[4], qij=parts[5], b=sub[6 * n], lam=sub[6 * n + 1],
+                    mu=sub[6 * n + 2])
+
+    @staticmethod
+    def heaviside(beta, eta, x_tilde):
+        """Heaviside projection (380) and its derivative (340)."""
+        L = _load()
+        xt = np.ascontiguousarray(x_tilde, dtype=np.float64)
+        xp, dx = np.zeros_like(xt), np.zeros_like(xt)
+        L.orc_heaviside(len(xt), float(beta), float(eta), xt, xp.ctypes.data, dx.ctypes.data)
+        return xp, dx
+
+    @staticmethod
+    def pnorm(xhat, vol_max):
+        """p-norm local-volume constraint value (360) and gradient (335)."""
+        L = _load()
+        xh = np.ascontiguousarray(xhat, dtype=np.float64)
+        vm = np.ascontiguousarray(vol_max, dtype=np.float64)
+        g = np.zeros_like(xh)
+        return L.orc_pnorm(len(xh), xh, vm, g), g
+
+    def compliance_sens(self, xPhys, ce, c_solid):
+        """dc = -DeSIMP(xPhys) .* ce / cSolid (129 / 330)."""
+        dc = np.zeros(self.nE)
+        self.L.orc_compliance_sens(self.h, np.ascontiguousarray(xPhys, dtype=np.float64),
+                                   np.ascontiguousarray(ce, dtype=np.float64), float(c_solid), dc)
+        return dc
+
+    def local_sens(self, beta, xPhys, xTilde, ve0, dc):
+        """The LOCAL sensitivity chain (334-350): returns (fval, dc filtered,
+        dfdx, filter CG iterations).  Filter slots 0 (rMin) and 1 (rHat)
+        must be set up."""
+        d = np.array(dc, dtype=np.float64)
+        dfdx = np.zeros(self.nE)
+        its = C.c_int(0)
+        f = self.L.orc_local_sens(self.h, float(beta), np.ascontiguousarray(xPhys, dtype=np.float64),
+                                  np.ascontiguousarray(xTilde, dtype=np.float64), float(ve0), d, dfdx,
+                                  C.byref(its))
+        return f, d, dfdx, its.value
 
     def run(self, mode, V0, nLoop, rMin, rHat=0.0, nSnap=0):
         nE = self.nE

@@ -27,9 +27,19 @@
 //   compliance .... sum E ce == u^T K u
 //   PDE filter .... scipy-assembled Helmholtz system, constants, self-adjoint
 //   OC ............ numpy brute force of the bisection
-//   Heaviside/pnorm closed forms, finite-difference gradient
-//   MMA ........... KKT residual + independent primal-dual MMA subproblem
-// Every function of this file is covered by one of those checks.
+//   Heaviside ..... closed forms (H(0) = 0, H(1) = 1, H(eta) = 1/2, slope at
+//                   eta), beta -> 0 / infinity limits, central differences
+//   p-norm ........ closed forms (uniform, one non-zero entry), degree-one
+//                   homogeneity, central differences of the gradient
+//   LOCAL chain ... dfdx and dc (330-350) against central differences of the
+//                   exact composition (dense filters, direct elastic solve)
+//   MMA ........... first-order agreement of the GenSub model with f and g at
+//                   xval, hand-computed asymptotes, KKT residual of the
+//                   returned (x, lambda), an independent nested-bisection
+//                   solve of the same subproblem, convergence to the analytic
+//                   optimum of a separable convex problem
+// (tests/test_oracle_pins.py, tests/test_oracle_local_pins.py).  Every
+// function of this file is covered by one of those checks.
 //
 // The only parts not pinnable without MATLAB: bitwise order of MATLAB's
 // multithreaded reductions / accumarray and CHOLMOD's ordering.  Those affect
@@ -1418,6 +1428,65 @@ struct RunOut {
 inline double pow16(double v) { double v2 = v * v, v4 = v2 * v2, v8 = v4 * v4; return v8 * v8; }
 inline double pow15(double v) { double v2 = v * v, v4 = v2 * v2, v8 = v4 * v4; return v8 * v4 * v2 * v; }
 
+// Compliance sensitivity with respect to the physical densities,
+// dc = -DeSIMP(xPhys) .* ce / cSolid (108, 129 / 330).
+void compliance_sensitivity(const Model& M, i64 n, const double* xPhys, const double* ce, double cSolid, double* dc) {
+  const double E0 = M.S.E0, Emin = M.S.Emin;
+  for (i64 e = 0; e < n; ++e) dc[e] = -(M.S.penal * (E0 - Emin) * (xPhys[e] * xPhys[e])) * (ce[e] / cSolid);
+}
+
+// Heaviside projection (380) and its derivative (340) at xTilde for one beta;
+// either output may be null.
+void heaviside(i64 n, double beta, double eta, const double* xt, double* xPhys, double* dx) {
+  const double t0 = std::tanh(beta * eta);
+  const double den = t0 + std::tanh(beta * (1 - eta));
+  for (i64 e = 0; e < n; ++e) {
+    const double th = std::tanh(beta * (xt[e] - eta));
+    if (xPhys) xPhys[e] = (t0 + th) / den;
+    if (dx) dx[e] = beta * (1 - th * th) / den;
+  }
+}
+
+// p-norm local volume with p = 16 (pPIO, 235): the constraint value
+// fval = mean((xhat/volMax)^p)^(1/p) - 1 (360) and its gradient with respect
+// to xhat (335).
+double pnorm_volume(i64 n, const double* xhat, const double* volMax, double* dfdx_pde) {
+  const double p = 16.0;
+  const double sp = det_sum(n, [&](i64 i) { return pow16(xhat[i]) / pow16(volMax[i]); }) / (double)n;
+  const double fac = std::pow(sp, 1.0 / p - 1.0);
+  for (i64 e = 0; e < n; ++e) dfdx_pde[e] = fac * (pow15(xhat[e]) / pow16(volMax[e])) / (double)n;  // 335
+  return std::pow(sp, 1.0 / p) - 1.0;  // 360
+}
+
+// The LOCAL-mode sensitivity chain of one design iteration (334-350), filters
+// in slots 0 (rMin) and 1 (rHat) set up:
+//   xhat = F_rHat(xPhys); fval, dfdx_pde (p-norm); dx = H'(xTilde);
+//   dc <- F_rMin(dc .* dx); dfdx = F_rMin(F_rHat(dfdx_pde) .* dx).
+// Returns fval; t[0] += local-volume seconds, t[1] += filtering seconds.
+double local_sensitivities(const Model& M, double beta, const double* xPhys, const double* xTilde,
+                           const double* volMax, double* dc, double* dfdx, int* fIts, double* t) {
+  const i64 nE = M.L[0].numElements;
+  const double eta = 0.5;  // etaPIO, 234
+  std::vector<double> xhat(nE), dfdx_pde(nE), dx(nE), tmp(nE);
+  double t1 = now_s();
+  *fIts += filter_run(M, M.filt[1], xPhys, xhat.data());  // 334
+  const double fval = pnorm_volume(nE, xhat.data(), volMax, dfdx_pde.data());
+  double t2 = now_s();
+  t[0] += t2 - t1;
+  heaviside(nE, beta, eta, xTilde, nullptr, dx.data());  // 340
+  for (i64 e = 0; e < nE; ++e) tmp[e] = dc[e] * dx[e];
+  *fIts += filter_run(M, M.filt[0], tmp.data(), dc);  // 341
+  double t3 = now_s();
+  t[1] += t3 - t2;
+  *fIts += filter_run(M, M.filt[1], dfdx_pde.data(), dfdx);  // 346
+  double t4 = now_s();
+  t[0] += t4 - t3;
+  for (i64 e = 0; e < nE; ++e) tmp[e] = dfdx[e] * dx[e];
+  *fIts += filter_run(M, M.filt[0], tmp.data(), dfdx);  // 350
+  t[1] += now_s() - t4;
+  return fval;
+}
+
 int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunOut& O) {
   const Level& L = M.L[0];
   const int nE = L.numElements;
@@ -1440,12 +1509,11 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
   const double cSolid = det_sum(nE, [&](i64 i) { return M.eleModulus[i] * ce[i]; });
   if (O.cSolid) *O.cSolid = cSolid;
   auto SIMP = [&](double xp) { return Emin + (xp * xp * xp) * (E0 - Emin); };       // 107
-  auto DeSIMP = [&](double xp) { return M.S.penal * (E0 - Emin) * (xp * xp); };     // 108
   double change = 1.0, sharp = 1.0, beta = 1.0;
   const double eta = 0.5;
   int loop = 0, loopbeta = 0;
   MMA mma;
-  std::vector<double> tmp(nE), dfdx(nE), dx(nE), xhat(nE), dfdx_pde(nE);
+  std::vector<double> tmp(nE), dfdx(nE);
   while (loop < nLoop && change > 0.0001 && sharp > 0.01) {  // 111 / 311
     const double t0 = now_s();
     loop++; loopbeta++;
@@ -1465,7 +1533,7 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
     double cObj = det_sum(nE, [&](i64 i) { return M.eleModulus[i] * (ce[i] / cSolid); });  // 125-126
     const double cDesign = cObj * cSolid;  // 127
     const double V = det_sum(nE, [&](i64 i) { return xPhys[i]; }) / (double)nE;  // 128
-    for (int e = 0; e < nE; ++e) dc[e] = -DeSIMP(xPhys[e]) * (ce[e] / cSolid);  // 129
+    compliance_sensitivity(M, nE, xPhys.data(), ce.data(), cSolid, dc.data());  // 129 / 330
     double tOpt = now_s() - tc, tFilt = 0.0, tLVF = 0.0, fval = 0.0;
     int fIts = 0;
     if (mode == 0) {
@@ -1484,30 +1552,12 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
       double t3 = now_s();
       tOpt += t3 - t2;
     } else {
-      const int p = 16;  // pPIO
-      double t1 = now_s();
-      fIts += filter_run(M, M.filt[1], xPhys.data(), xhat.data());  // 334
-      const double sp = det_sum(nE, [&](i64 i) { return pow16(xhat[i]) / pow16(volMax[i]); }) / (double)nE;
-      const double fac = std::pow(sp, 1.0 / p - 1.0);
-      for (int e = 0; e < nE; ++e) dfdx_pde[e] = fac * (pow15(xhat[e]) / pow16(volMax[e])) / (double)nE;  // 335
-      double t2 = now_s();
-      tLVF += t2 - t1;
-      const double den = std::tanh(beta * eta) + std::tanh(beta * (1 - eta));
-      for (int e = 0; e < nE; ++e) {  // 340
-        const double th = std::tanh(beta * (xTilde[e] - eta));
-        dx[e] = beta * (1 - th * th) / den;
-      }
-      for (int e = 0; e < nE; ++e) tmp[e] = dc[e] * dx[e];
-      fIts += filter_run(M, M.filt[0], tmp.data(), dc.data());  // 341
-      double t3 = now_s();
-      tFilt += t3 - t2;
-      fIts += filter_run(M, M.filt[1], dfdx_pde.data(), dfdx.data());  // 346
-      double t4 = now_s();
-      tLVF += t4 - t3;
-      for (int e = 0; e < nE; ++e) tmp[e] = dfdx[e] * dx[e];
-      fIts += filter_run(M, M.filt[0], tmp.data(), dfdx.data());  // 350
+      double tt[2] = {0.0, 0.0};
+      const double fvalP = local_sensitivities(M, beta, xPhys.data(), xTilde.data(), volMax.data(), dc.data(),
+                                               dfdx.data(), &fIts, tt);
+      tLVF += tt[0];
+      tFilt += tt[1];
       double t5 = now_s();
-      tFilt += t5 - t4;
       // MMA (354-369) over the active (non-passive) elements, 274-276
       std::vector<int> act;
       for (int e = 0; e < nE; ++e)
@@ -1516,7 +1566,7 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
       std::vector<double> xa(na), dca(na), dfa(na);
       for (int k = 0; k < na; ++k) { xa[k] = x[act[k]]; dca[k] = dc[act[k]]; dfa[k] = dfdx[act[k]]; }
       if (loop == 1) { mma.n = na; mma.xold1 = xa; mma.xold2 = xa; mma.low.assign(na, 0.0); mma.upp.assign(na, 0.0); }
-      fval = std::pow(sp, 1.0 / p) - 1.0;  // 360
+      fval = fvalP;  // 360
       std::vector<double> xmin(na), xmax(na);
       for (int k = 0; k < na; ++k) { xmin[k] = std::max(0.0, xa[k] - 0.1); xmax[k] = std::min(1.0, xa[k] + 0.1); }
       mma_update(mma, loop, xa.data(), xmin.data(), xmax.data(), dca.data(), fval, dfa.data(), nullptr);
@@ -1528,8 +1578,7 @@ int run(Model& M, int mode, double V0, int nLoop, double rMin, double rHat, RunO
       tOpt += t6 - t5;
       fIts += filter_run(M, M.filt[0], x.data(), xTilde.data());  // 374
       if (loopbeta >= 40) { beta = std::min(2 * beta, 128.0); loopbeta = 0; }  // 375-379
-      const double d2 = std::tanh(beta * eta) + std::tanh(beta * (1 - eta));
-      for (int e = 0; e < nE; ++e) xPhys[e] = (std::tanh(beta * eta) + std::tanh(beta * (xTilde[e] - eta))) / d2;  // 380
+      heaviside(nE, beta, eta, xTilde.data(), xPhys.data(), nullptr);  // 380
       if (!M.isPassive.empty())
         for (int e = 0; e < nE; ++e)
           if (M.isPassive[e]) xPhys[e] = 1.0;  // 381
@@ -1764,8 +1813,11 @@ double orc_oc(void* h, double* x, const double* xPhys, const double* dc, double 
 }
 // One MMA update (m = 1) on n variables.  State arrays (xold1, xold2, low,
 // upp) are read and written.  Returns the number of Newton steps.
-int orc_mma(int n, int iter, double* xval, const double* xmin, const double* xmax, double* xold1, double* xold2,
-            const double* dfdx, double gx, const double* dgdx, double* low, double* upp) {
+// sub (may be null): 6n + 3 doubles receiving the subproblem GenSub built
+// (alpha, beta, p0, q0, pij, qij: n each; then b) and the final dual
+// variables (lambda, mu) of SolveDIP.
+int orc_mma_ex(int n, int iter, double* xval, const double* xmin, const double* xmax, double* xold1, double* xold2,
+               const double* dfdx, double gx, const double* dgdx, double* low, double* upp, double* sub) {
   MMA S;
   S.n = n;
   S.xold1.assign(xold1, xold1 + n); S.xold2.assign(xold2, xold2 + n);
@@ -1777,7 +1829,47 @@ int orc_mma(int n, int iter, double* xval, const double* xmin, const double* xma
   std::memcpy(xold2, S.xold2.data(), n * sizeof(double));
   std::memcpy(low, S.low.data(), n * sizeof(double));
   std::memcpy(upp, S.upp.data(), n * sizeof(double));
+  if (sub) {
+    const std::vector<double>* parts[6] = {&S.alpha, &S.beta, &S.p0, &S.q0, &S.pij, &S.qij};
+    for (int k = 0; k < 6; ++k) std::memcpy(sub + (size_t)k * n, parts[k]->data(), n * sizeof(double));
+    sub[6 * (size_t)n] = S.b;
+    sub[6 * (size_t)n + 1] = S.lam;
+    sub[6 * (size_t)n + 2] = S.mu;
+  }
   return nn;
+}
+int orc_mma(int n, int iter, double* xval, const double* xmin, const double* xmax, double* xold1, double* xold2,
+            const double* dfdx, double gx, const double* dgdx, double* low, double* upp) {
+  return orc_mma_ex(n, iter, xval, xmin, xmax, xold1, xold2, dfdx, gx, dgdx, low, upp, nullptr);
+}
+
+// Heaviside projection (380) and derivative (340); xPhys or dx may be null.
+void orc_heaviside(int n, double beta, double eta, const double* xTilde, double* xPhys, double* dx) {
+  heaviside(n, beta, eta, xTilde, xPhys, dx);
+}
+// p-norm local-volume constraint (360) and its gradient in xhat (335).
+double orc_pnorm(int n, const double* xhat, const double* volMax, double* dfdx_pde) {
+  return pnorm_volume(n, xhat, volMax, dfdx_pde);
+}
+// dc = -DeSIMP(xPhys) .* ce / cSolid (129 / 330).
+int orc_compliance_sens(void* h, const double* xPhys, const double* ce, double cSolid, double* dc) {
+  Model* M = (Model*)h;
+  compliance_sensitivity(*M, M->L[0].numElements, xPhys, ce, cSolid, dc);
+  return 0;
+}
+// The LOCAL sensitivity chain (334-350) with filter slots 0 (rMin) and 1
+// (rHat) set up and volMax = Ve0 (278): dc is filtered in place, dfdx
+// receives the constraint gradient; returns fval (360).
+double orc_local_sens(void* h, double beta, const double* xPhys, const double* xTilde, double Ve0, double* dc,
+                      double* dfdx, int* filterIts) {
+  Model* M = (Model*)h;
+  const int nE = M->L[0].numElements;
+  std::vector<double> volMax(nE, Ve0);
+  double t[2] = {0.0, 0.0};
+  int its = 0;
+  const double f = local_sensitivities(*M, beta, xPhys, xTilde, volMax.data(), dc, dfdx, &its, t);
+  if (filterIts) *filterIts = its;
+  return f;
 }
 
 // Whole optimisation (mode 0 = GLOBAL / TOP3D_XL_TO, 1 = LOCAL / TOP3D_XL_PIO).
