@@ -57,7 +57,9 @@ typedef struct t3x_completion {
 } t3x_completion;
 
 int t3x_server_create(const t3x_server_config* cfg, t3x_server** out);
-/* Queues a request (returns at once).  T3X_ERR_INPUT if it can never fit the budget. */
+/* Queues a request (returns at once).  T3X_ERR_INPUT if it can never fit the
+ * budget, or if its tag names a request that is queued, running, or finished
+ * and not yet released (a tag is unique until t3x_server_release). */
 int t3x_server_submit(t3x_server* s, const t3x_request* req);
 /* Up to max finished requests not returned before; waits up to waitMs for the first. */
 int t3x_server_poll(t3x_server* s, t3x_completion* out, int max, int waitMs, int* n);

@@ -103,8 +103,17 @@ void cmd_create(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   t3x_problem* h = nullptr;
   int rc;
   if (explicitBC) {
-    // MATLAB's 1-based node ids -> the ABI's 0-based ids
+    // MATLAB's 1-based node ids -> the ABI's 0-based ids.  fixNodes / loadNodes
+    // are double vectors, fixState an nf x 3 and loadVec an nl x 3 double
+    // matrix: checked before any of them is read.
+    for (int k = 2; k <= 5; ++k)
+      if (!mxIsDouble(prhs[k]) || mxIsComplex(prhs[k]) || mxIsSparse(prhs[k]))
+        mexErrMsgIdAndTxt("t3x:input", "boundary conditions must be real full double arrays");
     const size_t nf = mxGetNumberOfElements(prhs[2]), nl = mxGetNumberOfElements(prhs[4]);
+    if (mxGetM(prhs[3]) != nf || mxGetN(prhs[3]) != 3)
+      mexErrMsgIdAndTxt("t3x:input", "fixState must be numel(fixNodes) x 3");
+    if (mxGetM(prhs[5]) != nl || mxGetN(prhs[5]) != 3)
+      mexErrMsgIdAndTxt("t3x:input", "loadVec must be numel(loadNodes) x 3");
     std::vector<int> fn(nf), ln(nl);
     std::vector<uint8_t> fs(3 * nf);
     std::vector<double> lv(3 * nl);
@@ -124,7 +133,8 @@ void cmd_create(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (nrhs >= 8 && mxIsCell(prhs[7])) {
       for (size_t k = 0; k < mxGetNumberOfElements(prhs[7]); ++k) {
         const mxArray* c = mxGetCell(prhs[7], k);
-        if (!c || mxGetN(c) != 4) mexErrMsgIdAndTxt("t3x:input", "Incompatible Mesh Data Format!");
+        if (!c || !mxIsDouble(c) || mxIsComplex(c) || mxIsSparse(c) || mxGetN(c) != 4)
+          mexErrMsgIdAndTxt("t3x:input", "Incompatible Mesh Data Format!");
         const size_t m = mxGetM(c);
         const double* a = mxGetDoubles(c);  // m x 4, column-major
         for (size_t i = 0; i < m; ++i) {

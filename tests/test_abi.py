@@ -95,3 +95,17 @@ def test_create_multi_checks_arguments_before_device():
         assert rc == 1, rc  # T3X_ERR_INPUT
         assert b"boundary-condition" in L.t3x_last_error(h)
         L.t3x_destroy(h)
+
+
+def test_footprint_refuses_planes_beyond_launch_limits():
+    """Kernels that write one reduction partial per work-group chunk z until
+    the grid fits the partials buffer; a node plane with more tiles than that
+    buffer even at one chunk (a 3840 x 3840 x 8 plate, which passes the 2^31
+    DOF check) is refused with T3X_ERR_INPUT instead of launching a broken
+    grid, while a 2048 x 2048 x 8 plate is accepted."""
+    import top3d_xl_amd as t3x
+    assert t3x.footprint(2048, 2048, 8) > 0
+    with pytest.raises(t3x.T3XError):
+        t3x.footprint(3840, 3840, 8)
+    # the same voxel count as a tall column: the y-x plane is small, z is chunked
+    assert t3x.footprint(8, 3840, 3840) > 0

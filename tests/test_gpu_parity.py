@@ -253,9 +253,10 @@ def test_config1_global_parity(oracle_lib):
 
 def test_config2_local_parity(oracle_lib):
     """Config 2 (true(50,100,50), LOCAL, local volume bound 0.5, rHat 6, rMin
-    sqrt 3): first 8 iterations, same bars and noise-floor rule as config 1;
-    the strict bar must hold at least over the first 3 iterations."""
-    _trajectory_parity("LOCAL", 0.5, 8, 6.0, 3)
+    sqrt 3): the first 10 iterations (the gated window), same bars and
+    noise-floor rule as config 1; the strict bar must hold at least over the
+    first 3 iterations."""
+    _trajectory_parity("LOCAL", 0.5, 10, 6.0, 3)
 
 
 @pytest.mark.parametrize("nu", [0.2, 0.45])

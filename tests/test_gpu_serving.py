@@ -92,18 +92,86 @@ def test_alone_equals_mix():
           f"peak admitted {peak / 2**20:.0f} MiB of budget {budget / 2**20:.0f} MiB")
 
 
-def test_served_request_equals_direct_run():
+PRECISION_FORM = dict(precision=1, vcycle=1, omega=0.4)
+
+
+@pytest.mark.parametrize("s,nloop,params", [(16, 5, {}), (48, 30, PRECISION_FORM)], ids=["fp64-16", "mixed-48"])
+def test_served_request_equals_direct_run(s, nloop, params):
     import top3d_xl_amd as t3x
-    S = t3x.Server(max_active=2)
-    S.submit(42, 16, 32, 16, "GLOBAL", 0.12, 5, math.sqrt(3), 0.0, (0.0, 0.0, -1.0), 1.0)
+    print(f"\n[gpu-serving] served vs direct s={s} nLoop={nloop} {params}", flush=True)
+    S = t3x.Server(max_active=2, **params)
+    S.submit(42, s, 2 * s, s, "GLOBAL", 0.12, nloop, math.sqrt(3), 0.0, (0.0, 0.0, -1.0), 1.0)
     got = []
     t0 = time.time()
-    while not got and time.time() - t0 < 60:
+    while not got and time.time() - t0 < 120:
         got = S.poll(wait_ms=100)
     S.close()
-    P = t3x.Problem(np.ones((16, 32, 16), bool))
-    r = P.run("GLOBAL", 0.12, 5, math.sqrt(3))
+    P = t3x.Problem(np.ones((s, 2 * s, s), bool), **params)
+    r = P.run("GLOBAL", 0.12, nloop, math.sqrt(3))
     P.close()
     assert got and got[0]["status"] == 0
     assert np.array_equal(got[0]["cHist"], r["cHist"])
     assert np.array_equal(got[0]["xPhys"], r["xPhys"])
+
+
+def test_config5_sizes_alone_equals_mix():
+    """The config-5 request mix at its real sizes and iteration counts in the
+    precision form: one request of each s in {48, 64, 96, 128, 192} plus one
+    at 32, modes and parameters from the seeded Poisson stream (GLOBAL nLoop
+    30, LOCAL nLoop 60), submitted at the stream's arrival times (rate 4/s)
+    under a memory budget that holds the two largest footprints but not all
+    six.  Each result must be bitwise the same as the request run alone."""
+    import top3d_xl_amd as t3x
+    from top3d_xl_amd.serve_bench import make_stream, run_stream
+    reqs = make_stream(6, 4.0, seed=9)
+    for r, s in zip(reqs, (192, 48, 128, 64, 96, 32)):
+        r["shape"] = (s, 2 * s, s)
+    print("\n[gpu-serving] config-5 sizes:", [(r["shape"][0], r["mode"], r["nLoop"]) for r in reqs], flush=True)
+    assert {r["mode"] for r in reqs} == {"GLOBAL", "LOCAL"}
+    fps = sorted(t3x.footprint(*r["shape"], mode=r["mode"], **PRECISION_FORM) for r in reqs)
+    budget = fps[-1] + fps[-2]
+    mix, wall_mix = run_stream(reqs, 4, mem_budget=budget, timeout=300.0, **PRECISION_FORM)
+    print(f"  mix: {len(mix)} done in {wall_mix:.1f}s", flush=True)
+    alone = {}
+    S = t3x.Server(max_active=1, **PRECISION_FORM)
+    for r in reqs:
+        S.submit(r["tag"], *r["shape"], r["mode"], r["V0"], r["nLoop"], math.sqrt(3), r["rHat"], r["load_dir"],
+                 r["load_mag"])
+        t0 = time.time()
+        while r["tag"] not in alone and time.time() - t0 < 120:
+            for c in S.poll(wait_ms=50):
+                alone[c["tag"]] = c
+    S.close()
+    assert len(alone) == len(mix) == len(reqs)
+    for r in reqs:
+        a, m = alone[r["tag"]], mix[r["tag"]]
+        print(f"  s={r['shape'][0]} {r['mode']}: loops {a['loops']} pcg {a['pcgIts']} c {a['cFinal']:.6e} "
+              f"alone {a['tEnd'] - a['tStart']:.2f}s, mix latency {m['tEnd'] - m['tSubmit']:.2f}s", flush=True)
+        assert a["status"] == m["status"] == 0, (a["status"], m["status"])
+        assert a["loops"] == m["loops"] and a["pcgIts"] == m["pcgIts"]
+        assert np.array_equal(a["cHist"], m["cHist"]), r["tag"]
+        assert np.array_equal(a["xPhys"], m["xPhys"]), r["tag"]
+
+
+def test_duplicate_tag_refused_until_released():
+    """A tag names one request until its completion is released: a second
+    submit with a live tag is refused, and accepted again after release."""
+    import top3d_xl_amd as t3x
+    S = t3x.Server(max_active=1)
+    try:
+        S.submit(7, 8, 16, 8, "GLOBAL", 0.3, 2)
+        with pytest.raises(t3x.T3XError):
+            S.submit(7, 8, 16, 8, "GLOBAL", 0.3, 2)
+        got = []
+        t0 = time.time()
+        while not got and time.time() - t0 < 60:
+            got = S.poll(wait_ms=100)  # poll() copies the results and releases the tag
+        assert got and got[0]["tag"] == 7 and got[0]["status"] == 0
+        S.submit(7, 8, 16, 8, "GLOBAL", 0.3, 2)
+        got = []
+        t0 = time.time()
+        while not got and time.time() - t0 < 60:
+            got = S.poll(wait_ms=100)
+        assert got and got[0]["tag"] == 7
+    finally:
+        S.close()

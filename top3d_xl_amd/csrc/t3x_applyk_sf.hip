@@ -336,12 +336,20 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   }
 }
 
+static long long sf_tiles_xy(int NY, int NX) {
+  return (long long)((NY + SF_NT - 1) / SF_NT) * ((NX + SF_NT - 1) / SF_NT);
+}
+
+// z chunk: doubled until the grid writes at most kMaxPartials partials; a
+// node plane with more tiles than that is refused at set-up (sf_plane_fits)
 static int sf_zc(const Geo& g) {
-  const long long bxy = (long long)((g.NY + SF_NT - 1) / SF_NT) * ((g.NX + SF_NT - 1) / SF_NT);
+  const long long bxy = sf_tiles_xy(g.NY, g.NX);
   int zc = SF_ZC;
-  while (bxy * ((g.NZ + zc - 1) / zc) > kMaxPartials) zc *= 2;
+  while (bxy * ((g.NZ + zc - 1) / zc) > kMaxPartials && zc < g.NZ) zc *= 2;
   return zc;
 }
+
+bool sf_plane_fits(int NY, int NX) { return sf_tiles_xy(NY, NX) <= kMaxPartials; }
 
 static dim3 sf_grid(const Geo& g) {
   const int zc = sf_zc(g);

@@ -324,6 +324,14 @@ std::vector<LevelDims> level_dims(int nely, int nelx, int nelz, long long nv, in
   return d;
 }
 
+// The fine node plane (NY x NX nodes; coarser planes are smaller) must fit the
+// launch shapes that write one reduction partial per work-group.
+bool plane_fits(int NY, int NX) {
+  return sf_plane_fits(NY, NX) && filter_plane_fits(NY, NX) && prolong_plane_fits(NY, NX);
+}
+const char* kPlaneTooLarge =
+    "model too large: the padded (nely+1) x (nelx+1) node plane exceeds the launch-shape limit (about 1.4e7 nodes)";
+
 void build_levels(t3x_problem& P, const uint8_t* vox) {
   long long nv = 0;
   const long long nvox = (long long)P.nely * P.nelx * P.nelz;
@@ -447,6 +455,10 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   const long long NN = F.NN;
   if (3 * NN >= (1LL << 31)) {  // the fine-level kernels index DOFs with 32-bit offsets
     P.err = "model too large: 3 x (nely+1)(nelx+1)(nelz+1) must stay below 2^31";
+    return T3X_ERR_INPUT;
+  }
+  if (!plane_fits(F.NY, F.NX)) {
+    P.err = kPlaneTooLarge;
     return T3X_ERR_INPUT;
   }
   // ---- boundary conditions (536-557 or explicit), 1294-1336
@@ -1488,6 +1500,8 @@ int t3x_footprint(const uint8_t* voxels, int nely, int nelx, int nelz, const t3x
     nv = 0;
     for (long long i = 0; i < (long long)nely * nelx * nelz; ++i) nv += voxels[i] ? 1 : 0;
   }
+  const LevelDims f = level_dims(nely, nelx, nelz, nv, P.coarsestControl)[0];
+  if (!plane_fits(f.ny + 1, f.nx + 1)) return T3X_ERR_INPUT;
   *bytes = footprint_estimate(nely, nelx, nelz, nv, P, mode);
   return T3X_OK;
 }

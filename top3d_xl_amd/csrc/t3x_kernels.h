@@ -39,6 +39,12 @@ struct SfCoef {
 };
 SfCoef sf_coefficients(double nu);
 int applyk_sf_blocks(const Geo& g);
+// Whether a level whose node grid has NY x NX nodes per plane fits the launch
+// shapes of the kernels that write one reduction partial per work-group
+// (at most kMaxPartials even with a single z chunk).
+bool sf_plane_fits(int NY, int NX);
+bool filter_plane_fits(int NY, int NX);
+bool prolong_plane_fits(int NY, int NX);
 void launch_apply_K_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                        const double* u, double* y, double* partials, const int* fl);
 void launch_fine_resid64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,

@@ -959,13 +959,18 @@ __global__ __launch_bounds__(kFaY * kFaX) void k_filter_apply(Geo g, int zc, con
     }
   }
 }
+// z chunk doubled until at most kMaxPartials work-groups; a node plane with
+// more columns than that is refused at set-up (filter_plane_fits)
 static dim3 filter_grid(const Geo& g, int& zc) {
   zc = 8;
   for (;;) {
     const dim3 d((g.NY + kFaY - 1) / kFaY, (g.NX + kFaX - 1) / kFaX, (g.NZ + zc - 1) / zc);
-    if ((long long)d.x * d.y * d.z <= kMaxPartials) return d;
+    if ((long long)d.x * d.y * d.z <= kMaxPartials || zc >= g.NZ) return d;
     zc *= 2;
   }
+}
+bool filter_plane_fits(int NY, int NX) {
+  return (long long)((NY + kFaY - 1) / kFaY) * ((NX + kFaX - 1) / kFaX) <= kMaxPartials;
 }
 void launch_filter_apply(hipStream_t s, const Geo& g, const double* kern, const uint8_t* emask, const double* u,
                          double* y, const RedWs* ws, double* sc, const int* fl) {

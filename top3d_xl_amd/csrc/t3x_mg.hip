@@ -197,11 +197,12 @@ void restrict_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const 
   }
 }
 
-// grid (y-x tiles, z chunks) and the planes per chunk; at most kMaxPartials blocks
+// grid (y-x tiles, z chunks) and the planes per chunk; at most kMaxPartials
+// blocks (a node plane beyond that is refused at set-up: prolong_plane_fits)
 dim3 prolong_grid(const Geo& gf, int* zc) {
   const int bx = (gf.NY * gf.NX + kBlock - 1) / kBlock;
   int c = 4;
-  while ((long long)bx * ((gf.NZ + c - 1) / c) > kMaxPartials) c *= 2;
+  while ((long long)bx * ((gf.NZ + c - 1) / c) > kMaxPartials && c < gf.NZ) c *= 2;
   *zc = c;
   return dim3(bx, (gf.NZ + c - 1) / c);
 }
@@ -302,6 +303,10 @@ void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, dou
                     const int* fl) {
   hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
                      n, fl);
+}
+
+bool prolong_plane_fits(int NY, int NX) {
+  return ((long long)NY * NX + kBlock - 1) / kBlock <= kMaxPartials;
 }
 
 }  // namespace t3x

@@ -12,6 +12,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -42,6 +43,7 @@ struct t3x_server {
   std::deque<Pending> queue;
   std::map<long long, Finished> done;  // by tag
   std::deque<long long> readyOrder;    // finished, not yet polled
+  std::set<long long> liveTags;        // queued, running or finished and not yet released
   std::vector<std::thread> workers;
   long long memInUse = 0;
   int running = 0;
@@ -171,6 +173,8 @@ int t3x_server_submit(t3x_server* s, const t3x_request* req) {
   {
     std::lock_guard<std::mutex> lk(s->m);
     if (s->stopping) return T3X_ERR_INPUT;
+    // a tag names one request until its completion is released
+    if (!s->liveTags.insert(req->tag).second) return T3X_ERR_INPUT;
     s->queue.push_back(Pending{*req, bytes, s->now()});
   }
   s->cvWork.notify_all();
@@ -200,6 +204,7 @@ int t3x_server_release(t3x_server* s, long long tag) {
   auto it = s->done.find(tag);
   if (it == s->done.end() || !it->second.returned) return T3X_ERR_INPUT;
   s->done.erase(it);
+  s->liveTags.erase(tag);
   return T3X_OK;
 }
 

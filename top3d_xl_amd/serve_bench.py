@@ -47,9 +47,11 @@ def make_stream(n, rate, seed=0, sizes=SIZES, nloop_scale=1.0):
     return reqs
 
 
-def run_stream(reqs, max_active, precision=0, mem_budget=0, timeout=3600.0):
+def run_stream(reqs, max_active, precision=0, mem_budget=0, timeout=3600.0, **params):
+    """Submits reqs at their arrival times; params: further t3x_params fields
+    (e.g. vcycle, omega) for every request."""
     import top3d_xl_amd as t3x
-    S = t3x.Server(max_active=max_active, mem_budget=mem_budget, precision=precision)
+    S = t3x.Server(max_active=max_active, mem_budget=mem_budget, precision=precision, **params)
     done = {}
     t0 = time.perf_counter()
     i = 0
