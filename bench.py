@@ -77,9 +77,7 @@ def make_model(args):
     return g["voxels"], g, 0.4
 
 
-def gpu_run(args, params, model):
-    import torch
-
+def gpu_run(args, params, model, sample_designs=False):
     import top3d_xl_amd as t3x
 
     vox, bc, V0 = model
@@ -94,8 +92,9 @@ def gpu_run(args, params, model):
     for _ in range(args.warmup):
         rec, _ = P.opt_step()
         warm.append(rec)
+    designs = [(len(warm), P.xphys())] if sample_designs else []
     P.kernel_time_reset()
-    torch.cuda.synchronize()
+    t3x.device_synchronize()
     t0 = time.perf_counter()
     recs = []
     for _ in range(args.steps):
@@ -103,8 +102,10 @@ def gpu_run(args, params, model):
         recs.append(rec)
         if fin:
             break
-    torch.cuda.synchronize()
+    t3x.device_synchronize()
     t1 = time.perf_counter()
+    if sample_designs:
+        designs.append((len(warm) + len(recs), P.xphys()))
     kn, kms = P.kernel_time(0)
     vn, vms = P.kernel_time(1)
     cn, cms = P.kernel_time(2)
@@ -118,24 +119,108 @@ def gpu_run(args, params, model):
                 c_hist=[r["c"] for r in recs], its_hist=[r["pcgIts"] for r in recs],
                 c_all=[r["c"] for r in warm + recs], its_all=[r["pcgIts"] for r in warm + recs],
                 t_rows=[r["t"] for r in recs], applyK=(kn, kms), vcycle=(vn, vms), coarse=(cn, cms),
-                assemble=(an, ams))
+                assemble=(an, ams), designs=designs)
 
 
-def cpu_baseline(args):
-    """FP64 oracle (C++/OpenMP restatement of TOP3D_XL.m) on a bounded sample."""
+def solve_accuracy(model, designs, params):
+    """Per-solve error of the precision form against the FP64 plain path on
+    identical inputs: for each sampled design of the precision form's run, the
+    elasticity system (SIMP modulus, 107/116; load F_, 1315-1321) is solved
+    from zero to the reference's tolerance (1e-3, 30) by both forms and to
+    1e-10 by the FP64 path (the converged displacement U*).  Compliance is
+    U^T K U = sum E ce (125-127); displacement errors are 2-norm relative."""
+    import numpy as np
+
+    import top3d_xl_amd as t3x
+    vox, bc, _ = model
+
+    def problem(**kw):
+        if bc is None:
+            return t3x.Problem(vox, **kw)
+        return t3x.Problem(vox, (bc["fix_nodes"], bc["fix_state"]), (bc["load_nodes"], bc["load_vec"]), **kw)
+
+    P64, Pm = problem(precision=0, maxIT=5000), problem(maxIT=5000, **params)
+    b = P64.load_vector()
+    out = []
+    for loop, x in designs:
+        E = 1e-6 + x ** 3 * (1.0 - 1e-6)
+        c_of = {}
+        for P in (P64, Pm):
+            P.set_modulus(E)
+            P.assemble()
+        U64, it64, _ = P64.pcg(b, tol=1e-3, maxit=5000)
+        Um, itm, _ = Pm.pcg(b, tol=1e-3, maxit=5000)
+        Us, its_, res_s = P64.pcg(b, x0=U64, tol=1e-10, maxit=5000)
+        for k, U in (("fp64", U64), ("mixed", Um), ("converged", Us)):
+            c_of[k] = float(E @ P64.compliance(U))
+        ns = float(np.linalg.norm(Us))
+        out.append({"design_iteration": loop, "pcg_its_fp64": it64, "pcg_its_mixed": itm,
+                    "compliance_rel_err_mixed_vs_fp64": abs(c_of["mixed"] - c_of["fp64"]) / c_of["fp64"],
+                    "displacement_rel_err_mixed_vs_fp64": float(np.linalg.norm(Um - U64) / np.linalg.norm(U64)),
+                    "compliance_rel_err_vs_converged": {"mixed": abs(c_of["mixed"] - c_of["converged"]) / c_of["converged"],
+                                                        "fp64": abs(c_of["fp64"] - c_of["converged"]) / c_of["converged"]},
+                    "displacement_rel_err_vs_converged": {"mixed": float(np.linalg.norm(Um - Us)) / ns,
+                                                          "fp64": float(np.linalg.norm(U64 - Us)) / ns},
+                    "converged_pcg_its": it64 + its_})
+        del U64, Um, Us
+    P64.close()
+    Pm.close()
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, fp64_its_per_iteration):
+    """FP64 oracle (C++/OpenMP restatement of TOP3D_XL.m) on bounded samples:
+    at 256x512x256 one assembly (919-1066) and a PCG capped at --cpu-pcg-its
+    iterations (600-639) on the uniform V0 design, extrapolated to a design
+    iteration with the FP64 path's mean PCG iteration count; and the
+    CPU-runnable 50x100x50 configuration for --cpu-iters design iterations."""
     import numpy as np
 
     from oracle import Oracle, build_oracle
     build_oracle()
-    vox = np.ones((50, 100, 50), dtype=bool)
-    o = Oracle(vox)
-    r = o.run("GLOBAL", 0.12, args.cpu_iters, math.sqrt(3))
-    its = int(r["lssIts"].sum())
-    cg = float(r["tHist"][:, 1].sum())
-    return dict(value=3 * o.nN * its / cg, unit="DOF/s", cores=Oracle.num_threads(), kind="port",
-                sample=f"MG-PCG of 50x100x50 (the CPU-runnable configuration), {r['loops']} GLOBAL design "
-                       f"iterations, {its} PCG iterations, FP64; ms per design iteration "
-                       f"{1e3 * float(r['tHist'][:, 5].mean()):.0f}")
+    ny, nx, nz = parse_size(args.size)
+    o = Oracle(np.ones((ny, nx, nz), dtype=bool))
+    o.set_modulus(np.full(o.nE, 1e-6 + 0.12 ** 3 * (1 - 1e-6)))  # SIMP of x = V0 (107, 116)
+    t0 = time.perf_counter()
+    o.assemble()
+    t1 = time.perf_counter()
+    b = o.get("F")
+    _, its, _ = o.pcg(b, tol=1e-30, maxit=args.cpu_pcg_its)
+    t2 = time.perf_counter()
+    del o, b
+    # the capped PCG ran its set-up (one K*u and one V-cycle, 609-612) and
+    # `its` iterations (one K*p and one V-cycle each, 614-629)
+    ms_pcg = 1e3 * (t2 - t1) / (its + 1)
+    dof = 3 * (ny + 1) * (nx + 1) * (nz + 1)
+    ms_iter = 1e3 * (t1 - t0) + ms_pcg * fp64_its_per_iteration
+    small = Oracle(np.ones((50, 100, 50), dtype=bool))
+    r = small.run("GLOBAL", 0.12, args.cpu_iters, math.sqrt(3))
+    its1 = int(r["lssIts"].sum())
+    cg1 = float(r["tHist"][:, 1].sum())
+    return dict(value=round(dof / (ms_pcg * 1e-3), 1), unit="DOF/s", cores=Oracle.num_threads(), kind="port",
+                cpu=cpu_model(),
+                sample=f"FP64 MG-PCG ('Adapted' V-cycle) of the {args.size} cuboid on the uniform V0 design: one "
+                       f"assembly ({1e3 * (t1 - t0):.0f} ms) and a PCG capped at {its} iterations "
+                       f"({ms_pcg:.0f} ms per PCG iteration, set-up counted as one)",
+                ms_per_pcg_iteration=round(ms_pcg, 1), assemble_ms=round(1e3 * (t1 - t0), 1),
+                ms_per_design_iteration_extrapolated=round(ms_iter, 1),
+                extrapolation=f"assembly + {fp64_its_per_iteration:.1f} PCG iterations (the FP64 path's mean per "
+                              f"design iteration in the timed window) x ms per PCG iteration; filter and OC "
+                              f"not included",
+                config1_dof_per_s=round(3 * small.nN * its1 / cg1, 1),
+                config1_sample=f"MG-PCG of 50x100x50, {r['loops']} GLOBAL design iterations, {its1} PCG "
+                               f"iterations; ms per design iteration {1e3 * float(r['tHist'][:, 5].mean()):.0f}")
 
 
 def main():
@@ -150,7 +235,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--check-every", type=int, default=4)
     ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-pcg-its", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the per-solve error check of the precision form")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -166,7 +253,7 @@ def main():
              "both": ["fp64", "mixed", "mixed_adapted", "fp64_standard"]}[args.precision]
     for name in names:
         print(f"[bench] {name}: {VARIANTS[name]}", file=sys.stderr, flush=True)
-        runs[name] = gpu_run(args, VARIANTS[name], model)
+        runs[name] = gpu_run(args, VARIANTS[name], model, sample_designs=(name == "mixed" and not args.no_accuracy))
     head = "mixed" if "mixed" in runs else "fp64"
     g = runs[head]
     ny, nx, nz = parse_size(args.size)
@@ -250,8 +337,16 @@ def main():
     if "fp64_standard" in runs and "mixed" in runs:
         a, b = runs["fp64_standard"], runs["mixed"]
         out["mg_pcg_speedup_mixed_vs_fp64_standard"] = round((a["cg_s"] / a["steps"]) / (b["cg_s"] / b["steps"]), 3)
+    if "mixed" in runs and runs["mixed"]["designs"]:
+        print("[bench] per-solve accuracy of the precision form", file=sys.stderr, flush=True)
+        acc = solve_accuracy(model, runs["mixed"]["designs"], VARIANTS["mixed"])
+        out["precision_form_per_solve_error"] = acc
+        out["precision_form_max_compliance_rel_err_vs_fp64"] = max(a["compliance_rel_err_mixed_vs_fp64"] for a in acc)
+        out["precision_form_max_displacement_rel_err_vs_fp64"] = max(a["displacement_rel_err_mixed_vs_fp64"]
+                                                                     for a in acc)
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
-        out["cpu_baseline"] = cpu_baseline(args)
+        ref = runs.get("fp64", g)
+        out["cpu_baseline"] = cpu_baseline(args, ref["pcg_its"] / ref["steps"])
     print(json.dumps(out))
 
 

@@ -212,6 +212,11 @@ int t3x_vcycle(t3x_problem* h, const double* r, double* x);
 int t3x_pcg(t3x_problem* h, const double* b, double* x, double tol, int maxIT, int* its, double* resnorm);
 /* TopOpti_ComputeUnitCompliance (579-598) */
 int t3x_compliance(t3x_problem* h, const double* U, double* ce);
+/* The load vector F_ (1315-1321; which = 0) or the displacement U_ of the
+ * last elasticity solve (121; which = 1; t3x_pcg's solution after a
+ * t3x_pcg call) of load case loadCase, compact nodal order (3 x numNodes,
+ * interleaved per node). */
+int t3x_get_nodal(t3x_problem* h, int which, int loadCase, double* out);
 /* TopOpti_SetupPDEfilter_matrixFree (816-868); slot 0 = rMin filter, 1 = rHat filter */
 int t3x_filter_setup(t3x_problem* h, int slot, double radius);
 /* TopOpti_ConductPDEFiltering_matrixFree (870-902) */

@@ -22,10 +22,16 @@
 // the cell moduli double-buffered in LDS, the next layer's global loads in
 // flight during the cell phase); one thread per cell forms E_e Ke u_e, then
 // each node thread adds the rows of its node from the 4 cells around its
-// column in a fixed order (deterministic, no atomics).  Boundary conditions:
-// fixed DOFs are zeroed where the cells read them (P_free u) and fixed rows
-// return u (703-715, 1338-1437).
+// column in a fixed order (deterministic, no atomics).  A thread's cells
+// of consecutive layers share a face: the x-y transform of that face and
+// the top-face output of the cell below stay in registers, so each layer
+// costs one face analysis and one face synthesis (about 150 flops per cell
+// instead of 196, a third fewer LDS reads).  Boundary conditions: fixed DOFs
+// are zeroed where the cells read them (P_free u) and fixed rows return u
+// (703-715, 1338-1437).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "t3x_kernels.h"
 #include "t3x_reduce.h"
@@ -37,43 +43,15 @@ namespace {
 constexpr int SF_NT = 15;          // node columns per tile edge
 constexpr int SF_CT = SF_NT + 1;   // cells per tile edge
 constexpr int SF_HT = SF_NT + 2;   // staged nodes per tile edge
-constexpr int SF_ZC = 16;          // node layers per work-group (doubled for huge grids; 16 measured
-                                   // faster than 32 and 64, level with 12 and 24)
+constexpr int SF_ZC = 40;          // node layers per work-group (at most; see sf_zc)
+constexpr int SF_ZC_MIN = 4;       // fewest node layers per work-group when splitting for parallelism
+constexpr int SF_MIN_GROUPS = 2048;  // work-groups below which the z chunks are shortened
 constexpr int SF_T = SF_CT * SF_CT;  // 256 threads: one per cell
 
 // Position bits of local node a in the transform: x = dx, y = 1 - dy (the
 // reference's y axis points up the rows), z = dz.
 __device__ __forceinline__ constexpr int sf_pos(int a) {
   return kLocDx[a] + 2 * (1 - kLocDy[a]) + 4 * kLocDz[a];
-}
-
-// In place: f[p] over positions p = x + 2y + 4z -> coefficients over
-// (sum, difference) per axis, same bit layout.
-template <typename T>
-__device__ __forceinline__ void sf_analyse(T f[8]) {
-#pragma unroll
-  for (int b = 1; b < 8; b <<= 1)
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (!(j & b)) {
-        const T s = f[j | b] + f[j], d = f[j | b] - f[j];
-        f[j] = s;
-        f[j | b] = d;
-      }
-}
-
-// Transpose of sf_analyse's differences: coefficients -> nodal values.
-template <typename T>
-__device__ __forceinline__ void sf_synthesise(T f[8]) {
-#pragma unroll
-  for (int b = 4; b >= 1; b >>= 1)
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (!(j & b)) {
-        const T lo = f[j] - f[j | b], hi = f[j] + f[j | b];
-        f[j] = lo;
-        f[j | b] = hi;
-      }
 }
 
 // Value of lane + 1 within its row of 16 lanes (DPP row_shl:1; the last lane
@@ -87,17 +65,46 @@ __device__ __forceinline__ double next_in_row(double v) {
   return __hiloint2double(hi, lo);
 }
 
-// E * Ke * u for one cell, all three components, in transform space.
-// c[0..7]: the constants for E = 1 (sf_coefficients), scaled here by e.
+// One face of a cell (4 nodes at positions p = x + 2y, the z = 0 half of
+// sf_pos) in place: values -> (sum, difference) coefficients along x, then
+// along y, same bit layout.  The z stage of the cell transform combines the
+// bottom and top faces' coefficients: (top + bottom, top - bottom).
 template <typename T>
-__device__ __forceinline__ void sf_cell(T X[8], T Y[8], T Z[8], const T* c, T e) {
-  sf_analyse(X);
-  sf_analyse(Y);
-  sf_analyse(Z);
+__device__ __forceinline__ void face_analyse(T f[4]) {
+#pragma unroll
+  for (int b = 1; b < 4; b <<= 1)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (!(j & b)) {
+        const T s = f[j | b] + f[j], d = f[j | b] - f[j];
+        f[j] = s;
+        f[j | b] = d;
+      }
+}
+
+// Transpose of face_analyse's differences: coefficients -> nodal values.
+template <typename T>
+__device__ __forceinline__ void face_synthesise(T f[4]) {
+#pragma unroll
+  for (int b = 2; b >= 1; b >>= 1)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (!(j & b)) {
+        const T lo = f[j] - f[j | b], hi = f[j] + f[j | b];
+        f[j] = lo;
+        f[j | b] = hi;
+      }
+}
+
+// E * W in transform space for one cell: analysed coefficients X, Y, Z of
+// the three components -> ox, oy, oz.  c[0..7]: the constants for E = 1
+// (sf_coefficients), scaled here by e.
+template <typename T>
+__device__ __forceinline__ void sf_couple(const T X[8], const T Y[8], const T Z[8], T ox[8], T oy[8], T oz[8],
+                                          const T* c, T e) {
   T k[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) k[i] = c[i] * e;
-  T ox[8], oy[8], oz[8];
   const T t = X[1] + Y[2] + Z[4];
   ox[0] = oy[0] = oz[0] = T(0);
   ox[1] = k[0] * X[1] + k[1] * t;
@@ -118,15 +125,6 @@ __device__ __forceinline__ void sf_cell(T X[8], T Y[8], T Z[8], const T* c, T e)
   ox[7] = k[7] * X[7];
   oy[7] = k[7] * Y[7];
   oz[7] = k[7] * Z[7];
-  sf_synthesise(ox);
-  sf_synthesise(oy);
-  sf_synthesise(oz);
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    X[i] = ox[i];
-    Y[i] = oy[i];
-    Z[i] = oz[i];
-  }
 }
 
 }  // namespace
@@ -156,14 +154,11 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   __shared__ T sE[2][SF_CT * SF_CT];      // [buffer][cell y + 16 cell x]
   __shared__ uint8_t sF[2][CS];           // fixation bits of the staged nodes
   // Gather of the cell rows into node rows: the y-neighbour pair is summed in
-  // registers across lanes and only the x-neighbour partial sums (6 rows) go
-  // through LDS; in FP64 that keeps the work-group at 30 KB of LDS (4 waves
-  // per SIMD; 2 with all 24 rows in LDS).
-  __shared__ T sR[6][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
+  // registers across lanes and only the x-neighbour partial sums (3 rows)
+  // go through LDS.
+  __shared__ T sR[3][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware tile order (xcd_tile_order): neighbouring tiles share an L2
-  // (measured at 256x512x256: FP64 K*p 0.88 -> 0.82 ms, FP32 smoothing step
-  // 1.04 -> 0.89 ms)
   const dim3 tile = xcd_tile_order();
   const int bx = tile.x, by = tile.y, bz = tile.z;
   const int y0 = bx * SF_NT, x0 = by * SF_NT;
@@ -177,9 +172,7 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   // staging: global loads a layer ahead into registers, LDS stores after the
   // cell phase has released the buffer.  u is staged unmasked beside its
   // fixation bits: the cells apply P_free when they read their nodes, and
-  // the node's own u and bits for the epilogue come from the same stage (no
-  // second global read; measured at 256x512x256: FP64 K*p 0.83 -> 0.71 ms,
-  // FP32 residual 0.73 -> 0.57 ms)
+  // the node's own u and bits for the epilogue come from the same stage.
   constexpr int NST = (CS + SF_T - 1) / SF_T;  // staged nodes per thread (2)
   T pu[NST][3], pe;
   uint8_t pf[NST];
@@ -227,13 +220,40 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   const bool gnode = ty < SF_NT && tx < SF_NT;
   const int yy = y0 + ty, xx = x0 + tx;
   const bool own = gnode && yy < g.NY && xx < g.NX;
-  T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0};
+  // the 4 nodes of cell (cy, cx) in one staged layer, P_free applied as
+  // read, face-analysed
+  auto read_face = [&](int b, T F[3][4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {  // local nodes 0..3: the z = 0 face
+      const int p = sf_pos(a);
+      const int h = (cx + kLocDx[a]) * SF_HT + (cy + kLocDy[a]);
+      const int o = (b * 3) * CS + h;
+      const uint8_t fa = sF[b][h];
+      F[0][p] = (fa & 1) ? T(0) : sU[o];
+      F[1][p] = (fa & 2) ? T(0) : sU[o + CS];
+      F[2][p] = (fa & 4) ? T(0) : sU[o + 2 * CS];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) face_analyse(F[c]);
+  };
   double pq = 0.0;
   const int kBeg = zc0 - 1;
   load_stage(kBeg);
   store_stage(kBeg & 1);
   load_stage(kBeg + 1);
   store_stage((kBeg + 1) & 1);
+  __syncthreads();
+  // The cell of layer k shares its bottom face with the top face of the cell
+  // of layer k - 1: the face's analysed coefficients (Fb) and the previous
+  // cell's top-face output coefficients (Ot) stay in registers, so a step
+  // reads and analyses one node layer and synthesises one (the sum of the
+  // two cells' coefficients on the shared face).
+  T Fb[3][4], Ot[3][4];
+  read_face(kBeg & 1, Fb);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Ot[c][j] = T(0);
   for (int k = kBeg; k < zc1; ++k) {
     const bool more = k + 2 <= zc1;
     if (more) load_stage(k + 2);  // in flight during this layer's cell phase
@@ -257,53 +277,61 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
     // (cy + 1, cx) in registers (q), and the pair at x + 1 comes through LDS
     // (sR) from the thread of column cx + 1, which formed the same sum for
     // the node at its x - 1.
-    T q[6];
+    T q[3];
     {
-      T X[8], Yv[8], Z[8];
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        const int p = sf_pos(a);
-        const int h = (cx + kLocDx[a]) * SF_HT + (cy + kLocDy[a]);
-        const int o = ((kLocDz[a] ? bk1 : bk) * 3) * CS + h;
-        const uint8_t fa = sF[kLocDz[a] ? bk1 : bk][h];
-        X[p] = (fa & 1) ? T(0) : sU[o];
-        Yv[p] = (fa & 2) ? T(0) : sU[o + CS];
-        Z[p] = (fa & 4) ? T(0) : sU[o + 2 * CS];
-      }
+      T Ft[3][4];
+      read_face(bk1, Ft);
       if (epi) {  // this node's bits and unmasked u, before its buffer is restaged
         const int h = (tx + 1) * SF_HT + (ty + 1);
         f = sF[bk][h];
 #pragma unroll
         for (int c = 0; c < 3; ++c) uo[c] = double(sU[(bk * 3 + c) * CS + h]);
       }
-      sf_cell(X, Yv, Z, cst, sE[bk][tid]);
-      // transform position of local (dy, dx, dz): dx + 2 (1 - dy) + 4 dz
+      T X[8], Yv[8], Z[8];
 #pragma unroll
-      for (int dz = 0; dz < 2; ++dz) {
-        const int p11 = 1 + 4 * dz, p01 = 3 + 4 * dz;  // (dy, dx) = (1, 1), (0, 1): this node column
-        const int p10 = 0 + 4 * dz, p00 = 2 + 4 * dz;  // (1, 0), (0, 0): the column at x - 1
-        q[3 * dz + 0] = X[p11] + next_in_row(X[p01]);
-        q[3 * dz + 1] = Yv[p11] + next_in_row(Yv[p01]);
-        q[3 * dz + 2] = Z[p11] + next_in_row(Z[p01]);
-        sR[3 * dz + 0][tid] = X[p10] + next_in_row(X[p00]);
-        sR[3 * dz + 1][tid] = Yv[p10] + next_in_row(Yv[p00]);
-        sR[3 * dz + 2][tid] = Z[p10] + next_in_row(Z[p00]);
+      for (int j = 0; j < 4; ++j) {  // z stage: positions j (sum) and j + 4 (difference)
+        X[j] = Ft[0][j] + Fb[0][j];
+        X[j + 4] = Ft[0][j] - Fb[0][j];
+        Yv[j] = Ft[1][j] + Fb[1][j];
+        Yv[j + 4] = Ft[1][j] - Fb[1][j];
+        Z[j] = Ft[2][j] + Fb[2][j];
+        Z[j + 4] = Ft[2][j] - Fb[2][j];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Fb[c][j] = Ft[c][j];
+      T ox[8], oy[8], oz[8];
+      sf_couple(X, Yv, Z, ox, oy, oz, cst, sE[bk][tid]);
+      // transposed z stage: the bottom face (node layer k) joins the previous
+      // cell's top face, then the face is synthesised once
+      T S[3][4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        S[0][j] = Ot[0][j] + (ox[j] - ox[j + 4]);
+        S[1][j] = Ot[1][j] + (oy[j] - oy[j + 4]);
+        S[2][j] = Ot[2][j] + (oz[j] - oz[j + 4]);
+        Ot[0][j] = ox[j] + ox[j + 4];
+        Ot[1][j] = oy[j] + oy[j + 4];
+        Ot[2][j] = oz[j] + oz[j + 4];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) face_synthesise(S[c]);
+      // face position of local (dy, dx): dx + 2 (1 - dy); (1, 1), (0, 1) are
+      // this node column, (1, 0), (0, 0) the column at x - 1
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        q[c] = S[c][1] + next_in_row(S[c][3]);
+        sR[c][tid] = S[c][0] + next_in_row(S[c][2]);
       }
     }
     __syncthreads();  // sR complete; node layer k's buffer free
     if (more) store_stage(bk);
-    // ---- gather: bottom rows (layer k) and top rows (layer k+1) of this node
-    if (gnode) {
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        a0[r] += q[r] + sR[r][tid + SF_CT];
-        a1[r] += q[3 + r] + sR[3 + r][tid + SF_CT];
-      }
-    }
     if (epi) {  // node layer k complete
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T ku = ((f >> c) & 1) ? T(uo[c]) : a0[c];
+        const T a = q[c] + sR[c][tid + SF_CT];
+        const T ku = ((f >> c) & 1) ? T(uo[c]) : a;
         if (EPI == SEPI_KU) {
           const TY v = TY(ku);
           y[c * NN + n] = v;
@@ -316,11 +344,6 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
           pq += double(v) * ro[c];
         }
       }
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      a0[r] = a1[r];
-      a1[r] = 0;
     }
   }
   if (reduce) {
@@ -340,13 +363,19 @@ static long long sf_tiles_xy(int NY, int NX) {
   return (long long)((NY + SF_NT - 1) / SF_NT) * ((NX + SF_NT - 1) / SF_NT);
 }
 
-// z chunk: doubled until the grid writes at most kMaxPartials partials; a
-// node plane with more tiles than that is refused at set-up (sf_plane_fits)
+// z chunk: the NZ node layers split into equal chunks of about SF_ZC
+// layers, more (down to SF_ZC_MIN layers) while the grid has fewer than
+// SF_MIN_GROUPS work-groups, fewer while it would write more than
+// kMaxPartials partials; a node plane with more tiles than that is refused
+// at set-up (sf_plane_fits)
 static int sf_zc(const Geo& g) {
   const long long bxy = sf_tiles_xy(g.NY, g.NX);
-  int zc = SF_ZC;
-  while (bxy * ((g.NZ + zc - 1) / zc) > kMaxPartials && zc < g.NZ) zc *= 2;
-  return zc;
+  long long nch = (g.NZ + SF_ZC - 1) / SF_ZC;
+  const long long want = (SF_MIN_GROUPS + bxy - 1) / bxy;
+  if (nch < want) nch = std::min<long long>(want, (g.NZ + SF_ZC_MIN - 1) / SF_ZC_MIN);
+  if (nch > kMaxPartials / bxy) nch = std::max<long long>(1, kMaxPartials / bxy);
+  if (nch < 1) nch = 1;
+  return (int)((g.NZ + nch - 1) / nch);
 }
 
 bool sf_plane_fits(int NY, int NX) { return sf_tiles_xy(NY, NX) <= kMaxPartials; }

@@ -1751,6 +1751,17 @@ int t3x_compliance(t3x_problem* h, const double* U, double* ce) {
   });
 }
 
+int t3x_get_nodal(t3x_problem* h, int which, int loadCase, double* out) {
+  return guarded(h, [&] {
+    if (!out || which < 0 || which > 1 || loadCase < 0 || loadCase >= h->nCases) {
+      h->err = "t3x_get_nodal: which must be 0 (F) or 1 (U) and loadCase within the load cases";
+      return (int)T3X_ERR_INPUT;
+    }
+    download_nodal(*h, which == 0 ? h->Fc[loadCase] : h->Uc[loadCase], out);
+    return (int)T3X_OK;
+  });
+}
+
 int t3x_filter_setup(t3x_problem* h, int slot, double radius) {
   return guarded(h, [&] {
     if (slot < 0 || slot > 1) return (int)T3X_ERR_INPUT;

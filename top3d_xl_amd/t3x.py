@@ -99,6 +99,7 @@ def load_library():
     L.t3x_vcycle.argtypes = [vp, _dp, _dp]
     L.t3x_pcg.argtypes = [vp, _dp, _dp, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.t3x_compliance.argtypes = [vp, _dp, _dp]
+    L.t3x_get_nodal.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.t3x_filter_setup.argtypes = [vp, C.c_int, C.c_double]
     L.t3x_filter.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
     L.t3x_oc.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -124,7 +125,7 @@ def load_library():
 
 EXPORTED = ["t3x_default_params", "t3x_create", "t3x_create_multi", "t3x_num_load_cases", "t3x_destroy", "t3x_last_error", "t3x_info", "t3x_run",
             "t3x_opt_begin", "t3x_opt_step", "t3x_opt_result", "t3x_kernel_time", "t3x_kernel_time_reset",
-            "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance",
+            "t3x_set_modulus", "t3x_assemble", "t3x_apply_K", "t3x_vcycle", "t3x_pcg", "t3x_compliance", "t3x_get_nodal",
             "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy", "t3x_set_passive",
             "t3x_set_passive_layers", "t3x_get_passive",
             "t3x_server_create", "t3x_server_submit", "t3x_server_poll", "t3x_server_release",
@@ -236,6 +237,25 @@ class Problem:
         ce = np.zeros(self.nE)
         self._check(self.L.t3x_compliance(self.h, np.ascontiguousarray(U, dtype=np.float64), ce))
         return ce
+
+    def load_vector(self, case=0):
+        """F_ of a load case (t3x_get_nodal), compact interleaved nodal order."""
+        out = np.zeros(3 * self.nN)
+        self._check(self.L.t3x_get_nodal(self.h, 0, int(case), out))
+        return out
+
+    def displacement(self, case=0):
+        """U_ of a load case after the last solve (t3x_get_nodal)."""
+        out = np.zeros(3 * self.nN)
+        self._check(self.L.t3x_get_nodal(self.h, 1, int(case), out))
+        return out
+
+    def xphys(self):
+        """The current physical densities (t3x_opt_result), compact cell order."""
+        x = np.zeros(self.nE)
+        R = Result(x.ctypes.data, None, None, None, None, None, None, None, 0, None, 0.0, 0, 0)
+        self._check(self.L.t3x_opt_result(self.h, C.byref(R)))
+        return x
 
     def filter_setup(self, slot, r):
         self._check(self.L.t3x_filter_setup(self.h, slot, float(r)))
@@ -393,6 +413,17 @@ def stream_copy_ms(nbytes, reps=20):
     return ms.value
 
 
+def device_synchronize():
+    """hipDeviceSynchronize of the HIP runtime libt3x is bound to (a caller
+    that also loads another copy of the runtime, e.g. through torch, would
+    otherwise wait on the wrong one)."""
+    L = load_library()
+    L.hipDeviceSynchronize.restype = C.c_int
+    rc = L.hipDeviceSynchronize()
+    if rc != 0:
+        raise T3XError(f"hipDeviceSynchronize failed ({rc})")
+
+
 class Server:
     """Continuous-admission serving engine (include/t3x_serve.h)."""
 
@@ -434,7 +465,7 @@ class Server:
         return dict(queued=q.value, running=r.value, ready=rd.value, mem_in_use=m.value)
 
     def close(self):
-        if self.h:
+        if getattr(self, "h", None):
             self.L.t3x_server_destroy(self.h)
             self.h = None
 
