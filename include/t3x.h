@@ -230,6 +230,24 @@ int t3x_mma(t3x_problem* h, int iter, double* xval, const double* xmin, const do
             double* xold2, const double* dfdx, double gx, const double* dgdx, double* low, double* upp,
             int* nNewton);
 
+/* ---- allocation-free entry point ------------------------------------------ */
+/* No counterpart in the reference: t3x_oc for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same OC bisection (145-155) on caller-owned
+ * DEVICE arrays of nCells compact design cells (x updated in place; xPhys,
+ * dc read; passive: NULL or one byte per cell, nonzero = held at 1), run on
+ * `stream` (a hipStream_t; NULL = the null stream) and waited for; *change
+ * and *nBisect are host outputs.  It calls no hipMalloc/hipFree: its
+ * reduction partials and bisection state live in a fixed device array of
+ * T3X_OC_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  Calls
+ * that overlap on one GPU share that array and race.  nCells beyond
+ * T3X_OC_NOALLOC_MAX_CELLS (the reference's int32 cell numbering) is refused
+ * with T3X_ERR_INPUT; err (errLen bytes, may be NULL) receives the message. */
+#define T3X_OC_NOALLOC_MAX_CELLS 2147483647LL
+#define T3X_OC_NOALLOC_SCRATCH_BYTES 262272
+int t3x_oc_noalloc(long long nCells, double* x, const double* xPhys, const double* dc, const uint8_t* passive,
+                   double V0, void* stream, double* change, int* nBisect, char* err, int errLen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libt3x.so")
-SOURCES = ["t3x_kernels.hip", "t3x_applyk.hip", "t3x_applyk_sf.hip", "t3x_mg.hip", "t3x_coarse.hip", "t3x_host.hip", "t3x_serve.hip"]
+SOURCES = ["t3x_kernels.hip", "t3x_applyk.hip", "t3x_applyk_sf.hip", "t3x_mg.hip", "t3x_coarse.hip", "t3x_host.hip", "t3x_serve.hip", "t3x_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

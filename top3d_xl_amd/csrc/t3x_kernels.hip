@@ -1037,13 +1037,13 @@ void launch_simp(hipStream_t s, long long n, const uint8_t* emask, const double*
 }
 
 // Masked sums over cells: mode 0: sum a; 1: sum a*b; 2: sum a*(1-a);
-// 3: sum a^16 / b^16 (p-norm, 335/360).
+// 3: sum a^16 / b^16 (p-norm, 335/360).  emask == nullptr: every cell.
 __global__ __launch_bounds__(kBlock) void k_masked_sum(long long n, const uint8_t* __restrict__ emask,
                                                        const double* __restrict__ a, const double* __restrict__ b,
                                                        int mode, RedWs ws, double* out) {
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (!emask[i]) continue;
+    if (emask && !emask[i]) continue;
     const double ai = a[i];
     switch (mode) {
       case 0: s += ai; break;
@@ -1069,7 +1069,8 @@ void launch_masked_sum(hipStream_t s, long long n, const uint8_t* emask, const d
 
 // ---- OC bisection (145-153), k-ary: one pass evaluates the 2^depth-1
 // midpoints of the next `depth` bisection levels; the last block replays the
-// sequential decisions with identical arithmetic.
+// sequential decisions with identical arithmetic.  emask == nullptr: every
+// cell is a design cell (compact cell arrays).
 // oc[]: 0 l1, 1 l2, 2 fval, 3 done, 4 lastLmid, 5 nBisect, 6 change
 constexpr int kOcDepth = 4;
 constexpr int kOcCand = (1 << kOcDepth) - 1;
@@ -1110,7 +1111,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_pass(long long n, const uint8_t* 
     rl[k] = 1.0 / sqrt(lm[k + 1]);
   }
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (!emask[i]) continue;
+    if (emask && !emask[i]) continue;
     const double xi = x[i];
     const double t = xi * sqrt(-dc[i]);
     const bool pas = passive && passive[i];
@@ -1163,7 +1164,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_final(long long n, const uint8_t*
   const double rl = 1.0 / sqrt(oc[4]);
   double m = 0.0;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (!emask[i]) continue;
+    if (emask && !emask[i]) continue;
     const double xi = x[i];
     const double xn = (passive && passive[i]) ? 1.0 : oc_clamp(xi, xi * sqrt(-dc[i]) * rl);
     m = fmax(m, fabs(xn - xi));

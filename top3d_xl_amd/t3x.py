@@ -119,6 +119,8 @@ def load_library():
     L.t3x_server_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_longlong)]
     L.t3x_server_destroy.argtypes = [vp]
+    L.t3x_oc_noalloc.argtypes = [C.c_longlong, vp, vp, vp, vp, C.c_double, vp, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int), C.c_char_p, C.c_int]
     _lib = L
     return L
 
@@ -129,7 +131,7 @@ EXPORTED = ["t3x_default_params", "t3x_create", "t3x_create_multi", "t3x_num_loa
             "t3x_filter_setup", "t3x_filter", "t3x_oc", "t3x_mma", "t3x_footprint", "t3x_memory", "t3x_stream_copy", "t3x_set_passive",
             "t3x_set_passive_layers", "t3x_get_passive",
             "t3x_server_create", "t3x_server_submit", "t3x_server_poll", "t3x_server_release",
-            "t3x_server_stats", "t3x_server_destroy"]
+            "t3x_server_stats", "t3x_server_destroy", "t3x_oc_noalloc"]
 
 
 def default_params(**kw) -> Params:
@@ -422,6 +424,19 @@ def device_synchronize():
     rc = L.hipDeviceSynchronize()
     if rc != 0:
         raise T3XError(f"hipDeviceSynchronize failed ({rc})")
+
+
+def oc_noalloc(n_cells, x_ptr, xphys_ptr, dc_ptr, passive_ptr, V0, stream=0):
+    """t3x_oc_noalloc on device arrays given by address (e.g. torch tensors'
+    data_ptr()): x updated in place on the device; returns (change, nBisect)."""
+    L = load_library()
+    ch, nb = C.c_double(0.0), C.c_int(0)
+    err = C.create_string_buffer(256)
+    rc = L.t3x_oc_noalloc(int(n_cells), x_ptr, xphys_ptr, dc_ptr, passive_ptr or None, float(V0), stream or None,
+                          C.byref(ch), C.byref(nb), err, len(err))
+    if rc != T3X_OK:
+        raise T3XError(f"t3x_oc_noalloc failed ({_STATUS.get(rc, rc)}): {err.value.decode()}")
+    return ch.value, nb.value
 
 
 class Server:
