@@ -168,6 +168,25 @@ def solve_accuracy(model, designs, params):
     return out
 
 
+def serving_leg(args):
+    """The serving workload (a seeded Poisson stream of config-5 requests:
+    cuboids (s, 2s, s), s over {32, ..., 192}, 70% GLOBAL nLoop 30 / 30%
+    LOCAL nLoop 60) through the engine in the precision form (4 slots,
+    continuous admission, pooled device memory) and through the one-at-a-time
+    FP64 baseline on the same stream: solves/s and p50/p99 latency."""
+    from top3d_xl_amd.serve_bench import make_stream, run_stream, summarize
+    reqs = make_stream(args.serve_requests, args.serve_rate, seed=args.seed, nloop_scale=args.serve_nloop_scale)
+    done, wall = run_stream(reqs, 4, **VARIANTS["mixed"])
+    print("[bench] serving: engine done", file=sys.stderr, flush=True)
+    done1, wall1 = run_stream(reqs, 1, precision=0)
+    print("[bench] serving: one-at-a-time FP64 done", file=sys.stderr, flush=True)
+    return {"requests": len(reqs), "rate_per_s": args.serve_rate, "nloop_scale": args.serve_nloop_scale,
+            "sizes_s": sorted({r["shape"][0] for r in reqs}),
+            "modes": {m: sum(1 for r in reqs if r["mode"] == m) for m in ("GLOBAL", "LOCAL")},
+            "engine": dict(max_active=4, precision="mixed 'Standard' (omega 0.4)", **summarize(reqs, done, wall)),
+            "one_at_a_time_fp64": summarize(reqs, done1, wall1)}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -238,6 +257,10 @@ def main():
     ap.add_argument("--cpu-pcg-its", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the per-solve error check of the precision form")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-workload leg")
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-nloop-scale", type=float, default=1.0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -344,6 +367,9 @@ def main():
         out["precision_form_max_compliance_rel_err_vs_fp64"] = max(a["compliance_rel_err_mixed_vs_fp64"] for a in acc)
         out["precision_form_max_displacement_rel_err_vs_fp64"] = max(a["displacement_rel_err_mixed_vs_fp64"]
                                                                      for a in acc)
+    if not args.no_serving and args.workload == "cuboid":
+        print("[bench] serving workload", file=sys.stderr, flush=True)
+        out["serving"] = serving_leg(args)
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         ref = runs.get("fp64", g)
         out["cpu_baseline"] = cpu_baseline(args, ref["pcg_its"] / ref["steps"])

@@ -27,7 +27,8 @@ enum { EPI_RESID = 1, EPI_SMOOTH = 2 };
 // cycle).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const T* __restrict__ Ks, const T* __restrict__ u,
-                                                      T* __restrict__ Yc, const int* fl) {
+                                                      T* __restrict__ Yc, const int* fl,
+                                                      const uint8_t* __restrict__ emask) {
   if (fl && fl[FL_DONE]) return;
   const int plane = g.NY * g.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -37,6 +38,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const T* __restrict
   if (cy >= g.ny || cx >= g.nx || cz >= g.nz) return;
   const long long NN = g.NN;
   const long long cell = idx + (long long)plane * cz;
+  if (!emask[cell]) {  // void cell: zero product without reading its (zero) matrix
+#pragma unroll
+    for (int i = 0; i < 24; ++i) Yc[(long long)i * NN + cell] = T(0);
+    return;
+  }
   T ue[24], ye[24];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
@@ -63,7 +69,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const T* __restrict
 template <typename T, int EPI>
 __global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const T* __restrict__ Yc, const T* __restrict__ u,
                                                            T* __restrict__ y, const T* __restrict__ rv,
-                                                           const T* __restrict__ dv, double omega, const int* fl) {
+                                                           const T* __restrict__ dv, double omega, const int* fl,
+                                                           const uint8_t* __restrict__ nmask) {
   if (fl && fl[FL_DONE]) return;
   const int plane = g.NY * g.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -72,6 +79,11 @@ __global__ __launch_bounds__(kBlock) void k_node_gather_Ks(Geo g, const T* __res
   const int z = blockIdx.y;
   const long long NN = g.NN;
   const long long n = idx + (long long)plane * z;
+  if (!nmask[n]) {  // void node: r, u and the result are 0
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c * NN + n] = T(0);
+    return;
+  }
   T acc[3] = {0, 0, 0};
 #pragma unroll
   for (int cc = 0; cc < 8; ++cc) {
@@ -100,35 +112,37 @@ long long coarse_ks_tmp_len(const Geo& g) { return 24LL * g.NN; }
 
 template <typename T>
 static void coarse_resid(hipStream_t s, const Geo& g, const T* Ks, const T* x, const T* r, T* d, T* tmp,
-                         const int* fl) {
-  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
+                         const int* fl, const uint8_t* emask, const uint8_t* nmask) {
+  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl, emask);
   hipLaunchKernelGGL((k_node_gather_Ks<T, EPI_RESID>), node_grid(g), dim3(kBlock), 0, s, g, (const T*)tmp, x, d, r,
-                     (const T*)nullptr, 0.0, fl);
+                     (const T*)nullptr, 0.0, fl, nmask);
 }
 
 template <typename T>
 static void coarse_smooth(hipStream_t s, const Geo& g, const T* Ks, const T* x, const T* r, const T* diag,
-                          double omega, T* out, T* tmp, const int* fl) {
-  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl);
+                          double omega, T* out, T* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask) {
+  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl, emask);
   hipLaunchKernelGGL((k_node_gather_Ks<T, EPI_SMOOTH>), node_grid(g), dim3(kBlock), 0, s, g, (const T*)tmp, x, out,
-                     r, diag, omega, fl);
+                     r, diag, omega, fl, nmask);
 }
 
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
-                           float* tmp, const int* fl) {
-  coarse_resid(s, g, Ks, x, r, d, tmp, fl);
+                           float* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask) {
+  coarse_resid(s, g, Ks, x, r, d, tmp, fl, emask, nmask);
 }
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
-                            const float* diag, double omega, float* out, float* tmp, const int* fl) {
-  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl);
+                            const float* diag, double omega, float* out, float* tmp, const int* fl,
+                            const uint8_t* emask, const uint8_t* nmask) {
+  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl, emask, nmask);
 }
 void launch_coarse_resid64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
-                           double* d, double* tmp, const int* fl) {
-  coarse_resid(s, g, Ks, x, r, d, tmp, fl);
+                           double* d, double* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask) {
+  coarse_resid(s, g, Ks, x, r, d, tmp, fl, emask, nmask);
 }
 void launch_coarse_smooth64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
-                            const double* diag, double omega, double* out, double* tmp, const int* fl) {
-  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl);
+                            const double* diag, double omega, double* out, double* tmp, const int* fl,
+                            const uint8_t* emask, const uint8_t* nmask) {
+  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl, emask, nmask);
 }
 
 }  // namespace t3x

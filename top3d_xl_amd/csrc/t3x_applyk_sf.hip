@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "t3x_kernels.h"
 #include "t3x_reduce.h"
@@ -44,6 +45,7 @@ constexpr int SF_NT = 15;          // node columns per tile edge
 constexpr int SF_CT = SF_NT + 1;   // cells per tile edge
 constexpr int SF_HT = SF_NT + 2;   // staged nodes per tile edge
 constexpr int SF_ZC = 40;          // node layers per work-group (at most; see sf_zc)
+constexpr int SF_ZC_SPARSE = 16;   // the same on domains with void tiles (shorter chunks: more tiles skipped)
 constexpr int SF_ZC_MIN = 4;       // fewest node layers per work-group when splitting for parallelism
 constexpr int SF_MIN_GROUPS = 2048;  // work-groups below which the z chunks are shortened
 constexpr int SF_T = SF_CT * SF_CT;  // 256 threads: one per cell
@@ -159,8 +161,18 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
   __shared__ T sR[3][SF_CT * SF_CT];  // [row][cell y + 16 cell x]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware tile order (xcd_tile_order): neighbouring tiles share an L2
+  // On a domain with void tiles the grid is the list of tiles that hold a
+  // solid cell (the others' owned nodes are void and stay 0).
   const dim3 tile = xcd_tile_order();
-  const int bx = tile.x, by = tile.y, bz = tile.z;
+  int bx = tile.x, by = tile.y, bz = tile.z;
+  const size_t tileId = bx + (size_t)gridDim.x * (by + (size_t)gridDim.y * bz);  // partials slot
+  if (g.tiles) {
+    const int lin = g.tiles[tileId];
+    const int tx = (g.NY + SF_NT - 1) / SF_NT, txy = tx * ((g.NX + SF_NT - 1) / SF_NT);
+    bz = lin / txy;
+    by = (lin - bz * txy) / tx;
+    bx = lin - bz * txy - by * tx;
+  }
   const int y0 = bx * SF_NT, x0 = by * SF_NT;
   const int zc0 = bz * zc;
   const int zc1 = min(g.NZ, zc0 + zc);
@@ -354,7 +366,7 @@ __global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, c
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < SF_T / 64; ++w) s += shp[w];
-      partials[bx + (size_t)gridDim.x * (by + (size_t)gridDim.y * bz)] = s;
+      partials[tileId] = s;
     }
   }
 }
@@ -368,9 +380,10 @@ static long long sf_tiles_xy(int NY, int NX) {
 // SF_MIN_GROUPS work-groups, fewer while it would write more than
 // kMaxPartials partials; a node plane with more tiles than that is refused
 // at set-up (sf_plane_fits)
-static int sf_zc(const Geo& g) {
+static int sf_zc(const Geo& g, bool sparse) {
   const long long bxy = sf_tiles_xy(g.NY, g.NX);
-  long long nch = (g.NZ + SF_ZC - 1) / SF_ZC;
+  const int zcMax = sparse ? SF_ZC_SPARSE : SF_ZC;
+  long long nch = (g.NZ + zcMax - 1) / zcMax;
   const long long want = (SF_MIN_GROUPS + bxy - 1) / bxy;
   if (nch < want) nch = std::min<long long>(want, (g.NZ + SF_ZC_MIN - 1) / SF_ZC_MIN);
   if (nch > kMaxPartials / bxy) nch = std::max<long long>(1, kMaxPartials / bxy);
@@ -380,49 +393,83 @@ static int sf_zc(const Geo& g) {
 
 bool sf_plane_fits(int NY, int NX) { return sf_tiles_xy(NY, NX) <= kMaxPartials; }
 
-static dim3 sf_grid(const Geo& g) {
-  const int zc = sf_zc(g);
+static dim3 sf_grid_full(const Geo& g, bool sparse) {
+  const int zc = sf_zc(g, sparse);
   return dim3((g.NY + SF_NT - 1) / SF_NT, (g.NX + SF_NT - 1) / SF_NT, (g.NZ + zc - 1) / zc);
 }
 
+// the launch shape of a level: the full tile grid, or one work-group per
+// listed tile (sparse chunking) when the level carries a tile list
+static dim3 sf_grid_of(const Geo& g) { return g.tiles ? dim3(std::max(g.nTiles, 1)) : sf_grid_full(g, false); }
+static int sf_zc_of(const Geo& g) { return sf_zc(g, g.tiles != nullptr); }
+
+std::vector<int> sf_active_tiles(const Geo& g, const std::vector<uint8_t>& emask) {
+  const dim3 d = sf_grid_full(g, true);
+  const int zc = sf_zc(g, true);
+  std::vector<uint8_t> f((size_t)d.x * d.y * d.z, 0);
+  // tile (bx, by, bz) stages cells y in [SF_NT bx - 1, SF_NT bx + SF_NT - 1],
+  // x likewise, z in [zc bz - 1, min(NZ, zc (bz + 1)) - 1]
+  auto range = [](int c, int step, int n, int& lo, int& hi) {
+    lo = std::min(n - 1, c / step);      // smallest b with step b + step - 1 >= c
+    hi = std::min(n - 1, (c + 1) / step);  // largest b with step b - 1 <= c
+  };
+  for (int z = 0; z < g.nz; ++z)
+    for (int x = 0; x < g.nx; ++x)
+      for (int y = 0; y < g.ny; ++y) {
+        if (!emask[y + (size_t)g.NY * (x + (size_t)g.NX * z)]) continue;
+        int y0, y1, x0, x1, z0, z1;
+        range(y, SF_NT, (int)d.x, y0, y1);
+        range(x, SF_NT, (int)d.y, x0, x1);
+        range(z, zc, (int)d.z, z0, z1);
+        for (int bz = z0; bz <= z1; ++bz)
+          for (int by = x0; by <= x1; ++by)
+            for (int bx = y0; bx <= y1; ++bx) f[bx + (size_t)d.x * (by + (size_t)d.y * bz)] = 1;
+      }
+  std::vector<int> list;
+  for (size_t t = 0; t < f.size(); ++t)
+    if (f[t]) list.push_back((int)t);
+  if (list.size() == f.size()) return {};
+  return list;
+}
+
 int applyk_sf_blocks(const Geo& g) {
-  const dim3 d = sf_grid(g);
+  const dim3 d = sf_grid_of(g);
   return (int)(d.x * d.y * d.z);
 }
 
 void launch_apply_K_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                        const double* u, double* y, double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_KU, double, double>), sf_grid(g), dim3(SF_T),
-                     0, s, g, sf_zc(g), cf, E, fix, u, y, partials, fl, partials ? 1 : 0, (const double*)nullptr,
+  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_KU, double, double>), sf_grid_of(g), dim3(SF_T),
+                     0, s, g, sf_zc_of(g), cf, E, fix, u, y, partials, fl, partials ? 1 : 0, (const double*)nullptr,
                      (const double*)nullptr, 0.0);
 }
 
 void launch_fine_resid64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                             const double* x, const double* r, double* d, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_RESID, double, double>), sf_grid(g),
-                     dim3(SF_T), 0, s, g, sf_zc(g), cf, E, fix, x, d, (double*)nullptr, fl, 0, r,
+  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_RESID, double, double>), sf_grid_of(g),
+                     dim3(SF_T), 0, s, g, sf_zc_of(g), cf, E, fix, x, d, (double*)nullptr, fl, 0, r,
                      (const double*)nullptr, 0.0);
 }
 
 void launch_fine_smooth64_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                              const double* x, const double* r, const double* diag, double omega, double* out,
                              double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_SMOOTH, double, double>), sf_grid(g),
-                     dim3(SF_T), 0, s, g, sf_zc(g), cf, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag,
+  hipLaunchKernelGGL((k_apply_K_sf<double, double, double, double, SEPI_SMOOTH, double, double>), sf_grid_of(g),
+                     dim3(SF_T), 0, s, g, sf_zc_of(g), cf, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag,
                      omega);
 }
 
 void launch_fine_resid32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const float* E, const uint8_t* fix,
                             const float* x, const double* r, float* d, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_sf<float, float, float, float, SEPI_RESID, double, float>), sf_grid(g), dim3(SF_T),
-                     0, s, g, sf_zc(g), cf, E, fix, x, d, (double*)nullptr, fl, 0, r, (const float*)nullptr, 0.0);
+  hipLaunchKernelGGL((k_apply_K_sf<float, float, float, float, SEPI_RESID, double, float>), sf_grid_of(g), dim3(SF_T),
+                     0, s, g, sf_zc_of(g), cf, E, fix, x, d, (double*)nullptr, fl, 0, r, (const float*)nullptr, 0.0);
 }
 
 void launch_fine_smooth32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const float* E, const uint8_t* fix,
                              const float* x, const double* r, const float* diag, double omega, float* out,
                              double* partials, const int* fl) {
-  hipLaunchKernelGGL((k_apply_K_sf<float, float, float, float, SEPI_SMOOTH, double, float>), sf_grid(g),
-                     dim3(SF_T), 0, s, g, sf_zc(g), cf, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag,
+  hipLaunchKernelGGL((k_apply_K_sf<float, float, float, float, SEPI_SMOOTH, double, float>), sf_grid_of(g),
+                     dim3(SF_T), 0, s, g, sf_zc_of(g), cf, E, fix, x, out, partials, fl, partials ? 1 : 0, r, diag,
                      omega);
 }
 

@@ -16,6 +16,7 @@
 
 #include "../../include/t3x.h"
 #include "t3x_coarse.h"
+#include "t3x_pool.h"
 #include "t3x_kernels.h"
 #include "t3x_reduce.h"
 
@@ -154,6 +155,13 @@ struct GroupTimer {
 };
 
 struct t3x_problem {
+  // Caller-provided resources (the serving engine's pooled slot): device
+  // allocations are carved in order out of [arena, arena + arenaSize) instead
+  // of hipMalloc'ed, the stream and the pinned read-back buffer are lent, and
+  // destroying the problem releases none of them.
+  char* arena = nullptr;
+  size_t arenaSize = 0, arenaUsed = 0;
+  bool ownStream = true, ownHostBuf = true;
   OptState opt;
   GroupTimer timers[TG_COUNT];
   t3x_params P{};
@@ -168,6 +176,8 @@ struct t3x_problem {
   SfCoef sf{};  // Ke of the unit brick as the constants of the sum-factorised K*u
   double* dT2 = nullptr;
   uint8_t* fix = nullptr;
+  uint8_t* act = nullptr;  // finest node chunks (16 nodes) with a solid node, or null when all are
+  Act fineAct() const { return Act{act, lv[0].NN}; }
   uint8_t* passive = nullptr;  // passive cells (passiveElements_), or null
   uint8_t* active = nullptr;   // solid and not passive (MMA design variables, 274-276)
   int nPassive = 0;
@@ -201,9 +211,16 @@ struct t3x_problem {
   CoarseSolver cs;
   long long* dCMap = nullptr;
   int* dNodePos = nullptr;
-  double *band = nullptr, *tinv = nullptr, *cPan = nullptr, *cDinv = nullptr, *cDT = nullptr, *cG = nullptr, *cH = nullptr,
-         *cva = nullptr, *cvb = nullptr;
-  int* dStatus = nullptr;
+  // the factorisation of the coarsest matrix: band (factored in place),
+  // scratch, and the solve operators (Dinv, DT, G, H)
+  struct CoarseFactor {
+    double *band = nullptr, *tinv = nullptr, *pan = nullptr, *Dinv = nullptr, *DT = nullptr, *G = nullptr,
+           *H = nullptr;
+    int* status = nullptr;
+  };
+  CoarseFactor cf;
+  const CoarseFactor& fac() const { return cf; }
+  double *cva = nullptr, *cvb = nullptr;
   bool assembled = false;
   // compact <-> dense maps (finest)
   int nNodes = 0, nCells = 0;
@@ -231,9 +248,20 @@ struct t3x_problem {
   long long bytes = 0;  // device bytes allocated by this problem
   template <class T>
   T* alloc(size_t n) {
+    const size_t nb = std::max<size_t>(n, 1) * sizeof(T);
+    if (arena) {
+      const size_t off = (arenaUsed + 255) & ~(size_t)255;
+      if (off + nb > arenaSize) throw HipError("device memory slot exhausted (footprint estimate exceeded)");
+      T* p = (T*)(arena + off);
+      T3X_HIP(hipMemsetAsync(p, 0, nb, stream));
+      T3X_HIP(hipStreamSynchronize(stream));
+      arenaUsed = off + nb;
+      bytes += (long long)nb;
+      return p;
+    }
     T* p = dalloc<T>(n, stream);
     allocs.push_back(p);
-    bytes += (long long)(std::max<size_t>(n, 1) * sizeof(T));
+    bytes += (long long)nb;
     return p;
   }
   hipEvent_t ev() {
@@ -276,9 +304,10 @@ struct t3x_problem {
       for (auto& pr : T.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
       for (auto e : T.pool) (void)hipEventDestroy(e);
     }
+    if (stream) (void)hipStreamSynchronize(stream);
     for (void* p : allocs) (void)hipFree(p);
-    if (hbuf) (void)hipHostFree(hbuf);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (hbuf && ownHostBuf) (void)hipHostFree(hbuf);
+    if (stream && ownStream) (void)hipStreamDestroy(stream);
   }
   const Level& f() const { return lv[0]; }
   double* xTilde() { return ca; }   // LOCAL-mode state between iterations
@@ -440,6 +469,8 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
   add(8 * (np / SB) * SB * SB, 4);                   // Dinv, DT, G, H
   add(8 * np, 2);                                    // solve vectors
   add(8LL * std::max(kMaxRedBlocks * 8, kMaxPartials) + 4096 * 4);  // reductions, scalars, flags
+  add(4LL * kMaxPartials);                           // K*u tile list (irregular domains)
+  add(NN / 16 + 1);                                  // node-chunk flags (irregular domains)
   return b + 4096 * nalloc;
 }
 
@@ -537,7 +568,7 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   if (nF == 0) { P.err = "No Fixations!"; return T3X_ERR_NO_FIX; }
   P.hLoaded = loaded;
 
-  T3X_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  if (!P.stream) T3X_HIP(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
   // ---- device arrays per level
   for (int l = 0; l < P.L; ++l) {
     Level& V = P.lv[l];
@@ -552,6 +583,24 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       V.Ks = P.alloc<double>((size_t)kTri * V.NN);
     }
     if (P.standard() && !P.mixed() && l >= 1 && l < P.L - 1) V.t = P.alloc<double>(3 * V.NN);
+    if (l == 0) {  // K*u work-groups only for the tiles that hold a solid cell
+      const std::vector<int> tl = sf_active_tiles(geo(V), V.hEmask);
+      if (!tl.empty()) {
+        V.sfTiles = P.alloc<int>(tl.size());
+        V.nSfTiles = (int)tl.size();
+        P.upload(V.sfTiles, tl.data(), tl.size() * sizeof(int));
+      }
+      // vector kernels skip 16-node chunks without a solid node
+      std::vector<uint8_t> ch((size_t)((V.NN + 15) / 16), 0);
+      bool all = true;
+      for (long long n = 0; n < V.NN; ++n)
+        if (V.hNmask[n]) ch[n >> 4] = 1;
+      for (uint8_t c : ch) all = all && c;
+      if (!all) {
+        P.act = P.alloc<uint8_t>(ch.size());
+        P.upload(P.act, ch.data(), ch.size());
+      }
+    }
     if (P.mixed()) {
       if (l < P.L - 1) V.diagf = P.alloc<float>(3 * V.NN);
       if (l >= 1) {
@@ -668,16 +717,17 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     P.dNodePos = P.alloc<int>(C.NN);
     P.upload(P.dNodePos, bestPos.data(), C.NN * sizeof(int));
     P.cs.plan(P.cN, P.cBW);
-    P.band = P.alloc<double>(P.cs.band_len());
-    P.tinv = P.alloc<double>(P.cs.tinv_len());
-    P.cPan = P.alloc<double>(P.cs.panel_len());
-    P.cDinv = P.alloc<double>(P.cs.blk_len());
-    P.cDT = P.alloc<double>(P.cs.blk_len());
-    P.cG = P.alloc<double>(P.cs.blk_len());
-    P.cH = P.alloc<double>(P.cs.blk_len());
+    auto& f = P.cf;
+    f.band = P.alloc<double>(P.cs.band_len());
+    f.tinv = P.alloc<double>(P.cs.tinv_len());
+    f.pan = P.alloc<double>(P.cs.panel_len());
+    f.Dinv = P.alloc<double>(P.cs.blk_len());
+    f.DT = P.alloc<double>(P.cs.blk_len());
+    f.G = P.alloc<double>(P.cs.blk_len());
+    f.H = P.alloc<double>(P.cs.blk_len());
+    f.status = P.alloc<int>(1);
     P.cva = P.alloc<double>(P.cs.np);
     P.cvb = P.alloc<double>(P.cs.np);
-    P.dStatus = P.alloc<int>(1);
   }
   // ---- compact maps (eleMapBack / nodMapBack order)
   for (long long n = 0; n < NN; ++n) {
@@ -698,7 +748,7 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
   P.oc = P.alloc<double>(8);
   P.mm = P.alloc<double>(MM_COUNT);
   P.dFiltKern = P.alloc<double>(128);
-  T3X_HIP(hipHostMalloc((void**)&P.hbuf, 256 * sizeof(double), hipHostMallocDefault));
+  if (!P.hbuf) T3X_HIP(hipHostMalloc((void**)&P.hbuf, 256 * sizeof(double), hipHostMallocDefault));
   return T3X_OK;
 }
 
@@ -728,12 +778,13 @@ int assemble(t3x_problem& P) {
     for (int l = 1; l < P.L - 1; ++l) launch_convert(s, P.lv[l].Ksf, P.lv[l].Ks, (long long)kTri * P.lv[l].NN);
   }
   const Level& C = P.lv[P.L - 1];
-  launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, P.band, (long long)P.cs.band_len());
-  P.cs.pad_identity(s, P.band);
-  launch_set_int(s, P.dStatus, 0);
-  P.cs.factor(s, P.band, P.tinv, P.cPan, P.cDinv, P.cDT, P.cG, P.cH, P.dStatus);
+  auto& f = P.cf;
+  launch_band_assemble(s, geo(C), C.Ks, C.emask, P.dNodePos, P.cBW, f.band, (long long)P.cs.band_len());
+  P.cs.pad_identity(s, f.band);
+  launch_set_int(s, f.status, 0);
+  P.cs.factor(s, f.band, f.tinv, f.pan, f.Dinv, f.DT, f.G, f.H, f.status);
   P.tend(TG_ASSEMBLE, ea);
-  T3X_HIP(hipMemcpyAsync(P.hbuf, P.dStatus, sizeof(int), hipMemcpyDeviceToHost, s));
+  T3X_HIP(hipMemcpyAsync(P.hbuf, f.status, sizeof(int), hipMemcpyDeviceToHost, s));
   P.sync();
   if (*(const int*)P.hbuf) { P.err = "coarsest matrix is not positive definite"; return T3X_ERR_NOT_PD; }
   P.assembled = true;
@@ -749,13 +800,13 @@ void vcycle(t3x_problem& P, const double* r, double* z, double* sc, const int* f
   double* tmp = (double*)P.rtmp;
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
-  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, tmp, P.lv[1].nmask, P.lv[1].r, fl);
+  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, tmp, P.lv[1].nmask, P.lv[1].r, fl, P.fineAct());
   for (int l = 1; l < L - 1; ++l)
     launch_restrict(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, (const double*)P.lv[l].r, tmp,
                     P.lv[l + 1].nmask, P.lv[l + 1].r, fl);
   Level& C = P.lv[L - 1];
   P.tbegin(TG_COARSE, ec);
-  P.cs.solve(s, P.dCMap, C.r, C.x, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);
+  P.cs.solve(s, P.dCMap, C.r, C.x, P.fac().Dinv, P.fac().DT, P.fac().G, P.fac().H, P.cva, P.cvb, fl);
   P.tend(TG_COARSE, ec);
   for (int l = L - 2; l >= 1; --l)
     launch_prolong_jacobi(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.P.omega,
@@ -774,13 +825,13 @@ void vcycle_mixed(t3x_problem& P, const double* r, float* z, double* sc, const i
   float* tmp = (float*)P.rtmp;
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
-  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
+  launch_restrict(s, geo(P.lv[0]), geo(P.lv[1]), P.lv[1].span, r, tmp, P.lv[1].nmask, P.lv[1].rf, fl, P.fineAct());
   for (int l = 1; l < L - 1; ++l)
     launch_restrict(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, (const float*)P.lv[l].rf, tmp,
                     P.lv[l + 1].nmask, P.lv[l + 1].rf, fl);
   Level& C = P.lv[L - 1];
   P.tbegin(TG_COARSE, ec);
-  P.cs.solve(s, P.dCMap, C.rf, C.xf, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);
+  P.cs.solve(s, P.dCMap, C.rf, C.xf, P.fac().Dinv, P.fac().DT, P.fac().G, P.fac().H, P.cva, P.cvb, fl);
   P.tend(TG_COARSE, ec);
   for (int l = L - 2; l >= 1; --l)
     launch_prolong_jacobi(s, geo(P.lv[l]), geo(P.lv[l + 1]), P.lv[l + 1].span, P.P.omega,
@@ -806,29 +857,30 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   const Geo g0 = geo(P.lv[0]);
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
-  if (!x0Ready) launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl);        // 757
+  if (!x0Ready) launch_jacobi0(s, r, P.lv[0].diagf, P.x32, om, 3 * P.lv[0].NN, fl, P.fineAct());  // 757
   launch_fine_resid32_sf(s, g0, P.sf, P.E32, P.fix, P.x32, r, P.d32, fl);                  // 761
-  launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const float*)P.d32, tmp, P.lv[1].nmask, P.lv[1].rf, fl);
+  launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const float*)P.d32, tmp, P.lv[1].nmask, P.lv[1].rf, fl,
+                  P.fineAct());
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
     launch_jacobi0(s, (const float*)V.rf, V.diagf, V.xf, om, 3 * V.NN, fl);
-    launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, tmp, fl);
+    launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, tmp, fl, V.emask, V.nmask);
     launch_restrict(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const float*)V.tf, tmp, P.lv[l + 1].nmask,
                     P.lv[l + 1].rf, fl);
   }
   Level& C = P.lv[L - 1];
   P.tbegin(TG_COARSE, ec);
-  P.cs.solve(s, P.dCMap, C.rf, C.xf, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);      // 767
+  P.cs.solve(s, P.dCMap, C.rf, C.xf, P.fac().Dinv, P.fac().DT, P.fac().G, P.fac().H, P.cva, P.cvb, fl);  // 767
   P.tend(TG_COARSE, ec);
   for (int l = L - 2; l >= 1; --l) {
     Level& V = P.lv[l];
     launch_prolong_add(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, P.lv[l + 1].xf, V.xf, V.nmask, fl);  // 770
-    launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, tmp, fl);     // 775
+    launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, tmp, fl, V.emask, V.nmask);  // 775
     std::swap(V.xf, V.tf);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, P.lv[1].xf, P.x32, P.lv[0].nmask, fl);
-  launch_fine_smooth32_sf(s, g0, P.sf, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z, fuse ? P.ws.partials : nullptr,
-                          fl);
+  launch_fine_smooth32_sf(s, g0, P.sf, P.E32, P.fix, P.x32, r, P.lv[0].diagf, om, z,
+                          fuse ? P.ws.partials : nullptr, fl);
   if (fuse) launch_fin1(s, fl ? FIN_ZR_LOOP : FIN_ZR_INIT, applyk_sf_blocks(g0), P.ws, sc, (int*)fl, nullptr);
   P.tend(TG_VCYCLE, ev);
 }
@@ -843,25 +895,26 @@ void vcycle_standard64(t3x_problem& P, const double* r, double* z, double* sc, c
   const Geo g0 = geo(P.lv[0]);
   hipEvent_t ev[2], ec[2];
   P.tbegin(TG_VCYCLE, ev);
-  launch_jacobi0(s, r, (const double*)P.lv[0].diag, P.x64, om, 3 * P.lv[0].NN, fl);          // 757
+  launch_jacobi0(s, r, (const double*)P.lv[0].diag, P.x64, om, 3 * P.lv[0].NN, fl, P.fineAct());  // 757
   launch_fine_resid64_sf(s, g0, P.sf, P.E, P.fix, P.x64, r, P.d64, fl);                      // 761
-  launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.d64, tmp, P.lv[1].nmask, P.lv[1].r, fl);
+  launch_restrict(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.d64, tmp, P.lv[1].nmask, P.lv[1].r, fl,
+                  P.fineAct());
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
     launch_jacobi0(s, (const double*)V.r, (const double*)V.diag, V.x, om, 3 * V.NN, fl);
-    launch_coarse_resid64(s, geo(V), V.Ks, V.x, V.r, V.t, tmp, fl);
+    launch_coarse_resid64(s, geo(V), V.Ks, V.x, V.r, V.t, tmp, fl, V.emask, V.nmask);
     launch_restrict(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const double*)V.t, tmp, P.lv[l + 1].nmask,
                     P.lv[l + 1].r, fl);
   }
   Level& C = P.lv[L - 1];
   P.tbegin(TG_COARSE, ec);
-  P.cs.solve(s, P.dCMap, C.r, C.x, P.cDinv, P.cDT, P.cG, P.cH, P.cva, P.cvb, fl);           // 767
+  P.cs.solve(s, P.dCMap, C.r, C.x, P.fac().Dinv, P.fac().DT, P.fac().G, P.fac().H, P.cva, P.cvb, fl);  // 767
   P.tend(TG_COARSE, ec);
   for (int l = L - 2; l >= 1; --l) {
     Level& V = P.lv[l];
     launch_prolong_add(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const double*)P.lv[l + 1].x, V.x, V.nmask,
                        fl);                                                                   // 770
-    launch_coarse_smooth64(s, geo(V), V.Ks, V.x, V.r, V.diag, om, V.t, tmp, fl);             // 775
+    launch_coarse_smooth64(s, geo(V), V.Ks, V.x, V.r, V.diag, om, V.t, tmp, fl, V.emask, V.nmask);  // 775
     std::swap(V.x, V.t);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, (const double*)P.lv[1].x, P.x64, P.lv[0].nmask, fl);
@@ -903,17 +956,17 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
     launch_apply_K(s, g0, P.sf, P.E, P.fix, P.p, P.q, P.ws, sc, fl, true);                  // 614-615
     P.tend(TG_APPLYK, ek);
     if (mx && P.standard())  // 617-622, with the first fine Jacobi step of 628-629 fused
-      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl, P.lv[0].diagf, P.x32, P.P.omega);
+      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl, P.lv[0].diagf, P.x32, P.P.omega, P.fineAct());
     else
-      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl);                             // 617-622
+      launch_pcg_update(s, y, P.r, P.p, P.q, n3, P.ws, sc, fl, nullptr, nullptr, 0.0, P.fineAct());  // 617-622
     if (mx) {
       if (P.standard()) vcycle_standard(P, P.r, P.z32, sc, fl, true, true);                // 628-629
       else vcycle_mixed(P, P.r, P.z32, sc, fl, true);
-      launch_pcg_pupdate(s, y, P.p, P.z32, n3, sc, fl);                                     // 616, 630
+      launch_pcg_pupdate(s, y, P.p, P.z32, n3, sc, fl, P.fineAct());                        // 616, 630
     } else {
       if (P.standard()) vcycle_standard64(P, P.r, P.z, sc, fl, true);                         // 628-629
       else vcycle(P, P.r, P.z, sc, fl, true);
-      launch_pcg_pupdate(s, y, P.p, P.z, n3, sc, fl);
+      launch_pcg_pupdate(s, y, P.p, P.z, n3, sc, fl, P.fineAct());
     }
   };
   // With params.graphs (and no kernel-group timing) the iteration is captured
@@ -954,7 +1007,7 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
       if (hf[FL_DONE]) { done = true; its = hf[FL_ITS]; break; }
     }
   }
-  launch_pcg_finish(s, y, P.p, n3, sc, fl);
+  launch_pcg_finish(s, y, P.p, n3, sc, fl, P.fineAct());
   T3X_HIP(hipMemcpyAsync(P.hbuf, sc, SC_COUNT_PCG * sizeof(double), hipMemcpyDeviceToHost, s));
   P.sync();
   if (resOut) *resOut = P.hbuf[SC_RES];
@@ -1018,9 +1071,9 @@ int filter_run(t3x_problem& P, int slot, const double* src, double* dst) {
   int its = maxIT;
   for (int it = 1; it <= maxIT; ++it) {
     launch_filter_apply(s, g0, kern, P.lv[0].emask, P.fp, P.fq, &P.ws, sc, fl);
-    launch_pcg_update(s, P.fx, P.fr, P.fp, P.fq, NN, P.ws, sc, fl);
+    launch_pcg_update(s, P.fx, P.fr, P.fp, P.fq, NN, P.ws, sc, fl, nullptr, nullptr, 0.0, P.fineAct());
     launch_scale_diag(s, P.fz, dinv, P.fr, NN, &P.ws, sc, fl);
-    launch_pcg_pupdate(s, P.fx, P.fp, P.fz, NN, sc, fl);
+    launch_pcg_pupdate(s, P.fx, P.fp, P.fz, NN, sc, fl, P.fineAct());
     if (it % ce == 0 || it == maxIT) {
       T3X_HIP(hipMemcpyAsync(P.hbuf, fl, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
       P.sync();
@@ -1028,7 +1081,7 @@ int filter_run(t3x_problem& P, int slot, const double* src, double* dst) {
       if (hf[FL_DONE]) { its = hf[FL_ITS]; break; }
     }
   }
-  launch_pcg_finish(s, P.fx, P.fp, NN, sc, fl);
+  launch_pcg_finish(s, P.fx, P.fp, NN, sc, fl, P.fineAct());
   launch_filter_n2e(s, g0, P.lv[0].emask, P.fx, dst);  // 894-901
   return its;
 }
@@ -1362,32 +1415,22 @@ int guarded(t3x_problem* h, F f) {
 
 }  // namespace
 
-// ================================================================ C ABI
-extern "C" {
-
-void t3x_default_params(t3x_params* p) {
-  p->E0 = 1.0;
-  p->nu = 0.3;
-  p->Emin = 1.0e-6;
-  p->penal = 3.0;
-  p->tol = 1.0e-3;
-  p->maxIT = 800;
-  p->omega = 0.6;
-  p->coarsestControl = 50000;
-  p->precision = 0;
-  p->checkEvery = 4;
-  p->timing = 0;
-  p->vcycle = 0;  // 'Adapted' (34)
-  p->graphs = 0;
-}
-
-int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
-                     const uint8_t* fixState, int nCases, const int* nPerCase, const int* loadNodes,
-                     const double* loadVec, const t3x_params* params, t3x_problem** out) {
+static int create_impl(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                       const uint8_t* fixState, int nCases, const int* nPerCase, const int* loadNodes,
+                       const double* loadVec, const t3x_params* params, const t3x::PoolSlot* slot,
+                       t3x_problem** out) {
   if (!out) return T3X_ERR_INPUT;
   *out = nullptr;
   std::unique_ptr<t3x_problem> P(new t3x_problem());
   if (params) P->P = *params; else t3x_default_params(&P->P);
+  if (slot) {  // pooled resources lent by the serving engine
+    P->arena = (char*)slot->base;
+    P->arenaSize = slot->size;
+    P->stream = slot->stream;
+    P->ownStream = false;
+    P->hbuf = slot->hostBuf;
+    P->ownHostBuf = false;
+  }
   P->nely = nely; P->nelx = nelx; P->nelz = nelz;
   if (!voxels || nelx < 3 || nely < 3 || nelz < 3) {  // 532-534
     P->err = "Inappropriate Input Model!";
@@ -1426,6 +1469,32 @@ int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nF
   }
   *out = P.release();
   return rc;
+}
+
+// ================================================================ C ABI
+extern "C" {
+
+void t3x_default_params(t3x_params* p) {
+  p->E0 = 1.0;
+  p->nu = 0.3;
+  p->Emin = 1.0e-6;
+  p->penal = 3.0;
+  p->tol = 1.0e-3;
+  p->maxIT = 800;
+  p->omega = 0.6;
+  p->coarsestControl = 50000;
+  p->precision = 0;
+  p->checkEvery = 4;
+  p->timing = 0;
+  p->vcycle = 0;  // 'Adapted' (34)
+  p->graphs = 0;
+}
+
+int t3x_create_multi(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                     const uint8_t* fixState, int nCases, const int* nPerCase, const int* loadNodes,
+                     const double* loadVec, const t3x_params* params, t3x_problem** out) {
+  return create_impl(voxels, nely, nelx, nelz, nFix, fixNodes, fixState, nCases, nPerCase, loadNodes, loadVec, params,
+                     nullptr, out);
 }
 
 int t3x_create(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
@@ -1821,3 +1890,12 @@ int t3x_mma(t3x_problem* h, int iter, double* xval, const double* xmin, const do
 }
 
 }  // extern "C"
+
+namespace t3x {
+int create_pooled(const uint8_t* voxels, int nely, int nelx, int nelz, int nFix, const int* fixNodes,
+                  const uint8_t* fixState, int nLoad, const int* loadNodes, const double* loadVec,
+                  const t3x_params* params, const PoolSlot& slot, t3x_problem** out) {
+  return create_impl(voxels, nely, nelx, nelz, nFix, fixNodes, fixState, 1, &nLoad, loadNodes, loadVec, params,
+                     &slot, out);
+}
+}  // namespace t3x

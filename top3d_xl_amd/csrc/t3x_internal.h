@@ -59,6 +59,8 @@ struct Level {
   float* tf = nullptr;       // FP32 'Standard' cycle: defect / smoother output (levels 1..L-2)
   float* diagf = nullptr;
   float* Ksf = nullptr;      // FP32 copy of Ks (levels 1..L-2, 'Standard' cycle)
+  int* sfTiles = nullptr;      // finest level: K*u tiles with a solid cell (null: all)
+  int nSfTiles = 0;
 };
 
 struct Filter {

@@ -12,13 +12,35 @@ enum ScalarSlot {
 };
 enum FlagSlot { FL_DONE = 0, FL_ITS, FL_ITER, FL_COUNT };  // <= 8
 
-// Geometry passed by value to kernels.
+// Geometry passed by value to kernels.  tiles / nTiles: the finest level's
+// sum-factorised K*u work-group tiles that hold a solid cell (linear ids in
+// the full tile grid, ascending), or null when every tile does; the K*u then
+// launches one work-group per listed tile.
 struct Geo {
   int ny, nx, nz;  // cells
   int NY, NX, NZ;  // nodes
   long long NN;
+  const int* tiles;
+  int nTiles;
 };
-inline Geo geo(const Level& L) { return Geo{L.ny, L.nx, L.nz, L.NY, L.NX, L.NZ, L.NN}; }
+inline Geo geo(const Level& L) {
+  return Geo{L.ny, L.nx, L.nz, L.NY, L.NX, L.NZ, L.NN, L.sfTiles, L.nSfTiles};
+}
+
+// Node chunks of a finest-level vector that hold a solid node: chunk[n >> 4]
+// for node n (16 nodes: one 128-byte line of a component in FP64), a nodal
+// 3-vector's element i standing for node i mod NN.  The vector kernels skip
+// void chunks (their entries are 0 and stay 0); chunk null: every chunk.
+struct Act {
+  const uint8_t* chunk = nullptr;
+  long long NN = 0;
+};
+__device__ __forceinline__ bool act_skip(const Act& a, long long i) {
+  if (!a.chunk) return false;
+  if (i >= a.NN) i -= a.NN;
+  if (i >= a.NN) i -= a.NN;
+  return !a.chunk[i >> 4];
+}
 
 // Reduction workspace: partials (kMaxRedBlocks * 8 doubles) + one counter.
 struct RedWs {
@@ -43,6 +65,10 @@ int applyk_sf_blocks(const Geo& g);
 // shapes of the kernels that write one reduction partial per work-group
 // (at most kMaxPartials even with a single z chunk).
 bool sf_plane_fits(int NY, int NX);
+// The K*u tiles whose staged cells include a solid one (linear ids in the
+// full tile grid of the sparse chunking, ascending); empty when all do.
+// g.tiles is ignored.
+std::vector<int> sf_active_tiles(const Geo& g, const std::vector<uint8_t>& emask);
 bool filter_plane_fits(int NY, int NX);
 bool prolong_plane_fits(int NY, int NX);
 void launch_apply_K_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
@@ -78,12 +104,14 @@ void launch_galerkin_coarse(hipStream_t s, const Geo& gf, const Geo& gc, const d
 // Separable restriction through an intermediate of restrict_tmp_len(gf, gc)
 // elements; prolongation + weighted Jacobi with optional fused z.r.
 long long restrict_tmp_len(const Geo& gf, const Geo& gc);
+// actF: the fine level's chunk flags (finest level only; the y pass reads no
+// fine value from a void chunk).
 void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, double* tmp,
-                     const uint8_t* nmaskC, double* rc, const int* fl);
+                     const uint8_t* nmaskC, double* rc, const int* fl, Act actF = {});
 void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, float* tmp,
-                     const uint8_t* nmaskC, float* rc, const int* fl);
+                     const uint8_t* nmaskC, float* rc, const int* fl, Act actF = {});
 void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* rf, float* tmp,
-                     const uint8_t* nmaskC, float* rc, const int* fl);
+                     const uint8_t* nmaskC, float* rc, const int* fl, Act actF = {});
 void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
                            const double* rf, const double* diag, const uint8_t* nmaskF, double* out,
                            const RedWs* ws, double* sc, const int* fl);
@@ -99,24 +127,28 @@ void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, c
 void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* xc, double* x,
                         const uint8_t* nmaskF, const int* fl);
 void launch_jacobi0(hipStream_t s, const double* r, const double* d, double* x, double omega, long long n,
-                    const int* fl);
+                    const int* fl, Act act = {});
 void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
-                    const int* fl);
+                    const int* fl, Act act = {});
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
-                    const int* fl);
+                    const int* fl, Act act = {});
 // K*u smoothers of coarse levels with the stored Galerkin matrices (FP32 copy
 // or the FP64 originals; t3x_applyk.hip).  resid: d = r - Kx; smooth:
 // out = x + omega (r - Kx) / diag.  tmp: coarse_ks_tmp_len(g) elements of
 // scratch (per-cell products).
 long long coarse_ks_tmp_len(const Geo& g);
+// emask, nmask: the level's cell and node masks (void cells read no matrix
+// entry, void nodes no product).
 void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r, float* d,
-                           float* tmp, const int* fl);
+                           float* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask);
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
-                            const float* diag, double omega, float* out, float* tmp, const int* fl);
+                            const float* diag, double omega, float* out, float* tmp, const int* fl,
+                            const uint8_t* emask, const uint8_t* nmask);
 void launch_coarse_resid64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
-                           double* d, double* tmp, const int* fl);
+                           double* d, double* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask);
 void launch_coarse_smooth64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
-                            const double* diag, double omega, double* out, double* tmp, const int* fl);
+                            const double* diag, double omega, double* out, double* tmp, const int* fl,
+                            const uint8_t* emask, const uint8_t* nmask);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const double* r, double* v, const int* fl);
 void launch_coarse_gather(hipStream_t s, int n, const long long* map, const float* r, double* v, const int* fl);
 void launch_coarse_scatter(hipStream_t s, int n, const long long* map, const double* v, double* x, long long NN3,
@@ -144,13 +176,14 @@ void launch_dot(hipStream_t s, const double* a, const double* b, long long n, co
 // the FP32 'Standard' V-cycle).  y and p are updated later by launch_pcg_pupdate.
 void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, const double* q, long long n,
                        const RedWs& ws, double* sc, int* fl, const float* dj = nullptr, float* xj = nullptr,
-                       double omega = 0.0);
+                       double omega = 0.0, Act act = {});
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
-                        const int* fl);
-void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl);
+                        const int* fl, Act act = {});
+void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl,
+                       Act act = {});
 // mixed precision: FP32 z = M r, FP64 y, r, p, q
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const float* z, long long n, const double* sc,
-                        const int* fl);
+                        const int* fl, Act act = {});
 void launch_stream_copy(hipStream_t s, double* dst, const double* src, long long nDoubles);  // nDoubles even
 void launch_convert(hipStream_t s, float* dst, const double* src, long long n);
 void launch_convert(hipStream_t s, double* dst, const float* src, long long n);

@@ -35,7 +35,7 @@ __device__ __forceinline__ double hat_w(int d) {
 // y pass: tmp[c][z][x][Jy] = sum_dy w(dy) rf[c](S*Jy+dy, x, z)
 template <int S, typename TI, typename TM>
 __global__ __launch_bounds__(kBlock) void k_restrict_y(Geo gf, int NYc, const TI* __restrict__ rf,
-                                                       TM* __restrict__ tmp, const int* fl) {
+                                                       TM* __restrict__ tmp, const int* fl, Act actF) {
   if (fl && fl[FL_DONE]) return;
   const int plane = NYc * gf.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -46,15 +46,20 @@ __global__ __launch_bounds__(kBlock) void k_restrict_y(Geo gf, int NYc, const TI
   const long long TN = (long long)plane * gf.NZ;
   const long long o = idx + (long long)plane * z;
   double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+  // the S * 2 - 1 fine nodes lie in at most two chunks; void ones are 0
+  const bool skip = actF.chunk && !actF.chunk[(base + max(S * Jy - (S - 1), 0)) >> 4] &&
+                    !actF.chunk[(base + min(S * Jy + S - 1, gf.NY - 1)) >> 4];
+  if (!skip) {
 #pragma unroll
-  for (int dy = -(S - 1); dy <= S - 1; ++dy) {
-    const int y = S * Jy + dy;
-    if (y < 0 || y >= gf.NY) continue;
-    const double w = hat_w<S>(dy);
-    const long long f = base + y;
-    t0 += w * (double)rf[f];
-    t1 += w * (double)rf[gf.NN + f];
-    t2 += w * (double)rf[2 * gf.NN + f];
+    for (int dy = -(S - 1); dy <= S - 1; ++dy) {
+      const int y = S * Jy + dy;
+      if (y < 0 || y >= gf.NY) continue;
+      const double w = hat_w<S>(dy);
+      const long long f = base + y;
+      t0 += w * (double)rf[f];
+      t1 += w * (double)rf[gf.NN + f];
+      t2 += w * (double)rf[2 * gf.NN + f];
+    }
   }
   tmp[o] = (TM)t0;
   tmp[TN + o] = (TM)t1;
@@ -161,16 +166,20 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
         p[1] += w * cv[c][1];
         p[2] += w * cv[c][2];
       }
+      // void nodes hold 0 in every level vector (all of their writers leave
+      // them 0): no load, no store
+      if (solid) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (ADD) {
-          out[k * NN + n] = solid ? (TO)((double)out[k * NN + n] + p[k]) : TO(0);
-        } else {
-          const double r = (double)rf[k * NN + n];
-          const double dd = solid ? (double)diag[k * NN + n] : 1.0;
-          const TO v = solid ? (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd) : TO(0);
-          zr += (double)v * r;
-          out[k * NN + n] = v;
+        for (int k = 0; k < 3; ++k) {
+          if (ADD) {
+            out[k * NN + n] = solid ? (TO)((double)out[k * NN + n] + p[k]) : TO(0);
+          } else {
+            const double r = (double)rf[k * NN + n];
+            const double dd = solid ? (double)diag[k * NN + n] : 1.0;
+            const TO v = solid ? (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd) : TO(0);
+            zr += (double)v * r;
+            out[k * NN + n] = v;
+          }
         }
       }
     }
@@ -184,15 +193,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
 
 template <typename TI, typename TM, typename TO>
 void restrict_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const TI* rf, TM* tmp,
-                   const uint8_t* nmaskC, TO* rc, const int* fl) {
+                   const uint8_t* nmaskC, TO* rc, const int* fl, Act actF) {
   const int plane = gc.NY * gf.NX;
   const dim3 g1((plane + kBlock - 1) / kBlock, gf.NZ);
   const dim3 g2((gc.NY * gc.NX + kBlock - 1) / kBlock, gc.NZ);
   if (span == 4) {
-    hipLaunchKernelGGL((k_restrict_y<4, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl);
+    hipLaunchKernelGGL((k_restrict_y<4, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl, actF);
     hipLaunchKernelGGL((k_restrict_xz<4, TM, TO>), g2, dim3(kBlock), 0, s, gf, gc, tmp, nmaskC, rc, fl);
   } else {
-    hipLaunchKernelGGL((k_restrict_y<2, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl);
+    hipLaunchKernelGGL((k_restrict_y<2, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl, actF);
     hipLaunchKernelGGL((k_restrict_xz<2, TM, TO>), g2, dim3(kBlock), 0, s, gf, gc, tmp, nmaskC, rc, fl);
   }
 }
@@ -227,9 +236,11 @@ void prolong_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, double 
 // x = omega * r / d on solid nodes (the 'Standard' cycle's pre-smoothing, 757)
 template <typename TR, typename TD, typename TO>
 __global__ __launch_bounds__(kBlock) void k_jacobi0(const TR* __restrict__ r, const TD* __restrict__ d,
-                                                    TO* __restrict__ x, double omega, long long n, const int* fl) {
+                                                    TO* __restrict__ x, double omega, long long n, const int* fl,
+                                                    Act act) {
   if (fl && fl[FL_DONE]) return;
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+    if (act_skip(act, i)) continue;
     const double dd = (double)d[i];
     x[i] = dd != 0.0 ? (TO)(omega * ((double)r[i] / dd)) : TO(0);
   }
@@ -240,16 +251,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi0(const TR* __restrict__ r, co
 long long restrict_tmp_len(const Geo& gf, const Geo& gc) { return 3LL * gc.NY * gf.NX * gf.NZ; }
 
 void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, double* tmp,
-                     const uint8_t* nmaskC, double* rc, const int* fl) {
-  restrict_impl<double, double, double>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl);
+                     const uint8_t* nmaskC, double* rc, const int* fl, Act actF) {
+  restrict_impl<double, double, double>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl, actF);
 }
 void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const double* rf, float* tmp,
-                     const uint8_t* nmaskC, float* rc, const int* fl) {
-  restrict_impl<double, float, float>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl);
+                     const uint8_t* nmaskC, float* rc, const int* fl, Act actF) {
+  restrict_impl<double, float, float>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl, actF);
 }
 void launch_restrict(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* rf, float* tmp,
-                     const uint8_t* nmaskC, float* rc, const int* fl) {
-  restrict_impl<float, float, float>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl);
+                     const uint8_t* nmaskC, float* rc, const int* fl, Act actF) {
+  restrict_impl<float, float, float>(s, gf, gc, span, rf, tmp, nmaskC, rc, fl, actF);
 }
 
 void launch_prolong_jacobi(hipStream_t s, const Geo& gf, const Geo& gc, int span, double omega, const double* xc,
@@ -290,19 +301,19 @@ void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, c
 }
 
 void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
-                    const int* fl) {
+                    const int* fl, Act act) {
   hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
-                     n, fl);
+                     n, fl, act);
 }
 void launch_jacobi0(hipStream_t s, const double* r, const double* d, double* x, double omega, long long n,
-                    const int* fl) {
+                    const int* fl, Act act) {
   hipLaunchKernelGGL((k_jacobi0<double, double, double>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x,
-                     omega, n, fl);
+                     omega, n, fl, act);
 }
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
-                    const int* fl) {
+                    const int* fl, Act act) {
   hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
-                     n, fl);
+                     n, fl, act);
 }
 
 bool prolong_plane_fits(int NY, int NX) {

@@ -1,7 +1,14 @@
 // Serving engine of libt3x: continuous admission of topology optimisations
-// by device footprint, one worker thread + one t3x_problem (own HIP stream)
-// per running request.  Host code only; the device work is the same as a
-// t3x_run call's.
+// by device footprint, one worker thread + one t3x_problem per running
+// request.  Host code only; the device work is the same as a t3x_run call's.
+//
+// Device memory is one pool allocated when the server starts (the budget):
+// an admitted request gets a contiguous range of its footprint (first fit,
+// ranges coalesced when released) and its problem carves its arrays out of
+// it, so serving makes no hipMalloc or hipFree per request (a hipFree can
+// wait for the whole device, including the other requests' work).  Each
+// worker keeps one HIP stream and one pinned read-back buffer for all the
+// requests it runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,6 +17,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <mutex>
 #include <set>
@@ -18,6 +26,7 @@
 
 #include "../../include/t3x.h"
 #include "../../include/t3x_serve.h"
+#include "t3x_pool.h"
 
 namespace {
 
@@ -25,6 +34,13 @@ struct Pending {
   t3x_request req;
   long long bytes;
   double tSubmit;
+  size_t offset = 0;  // the pool range, once admitted
+};
+
+// Worker-owned resources reused by every request the worker runs.
+struct WorkerRes {
+  hipStream_t stream = nullptr;
+  double* hostBuf = nullptr;
 };
 
 struct Finished {
@@ -48,13 +64,44 @@ struct t3x_server {
   long long memInUse = 0;
   int running = 0;
   bool stopping = false;
+  char* pool = nullptr;                 // memBudget bytes of device memory
+  std::map<size_t, size_t> freeRanges;  // offset -> size, disjoint, coalesced
 
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
   void worker();
-  void run_one(const Pending& p, Finished& f);
+  void run_one(const Pending& p, const WorkerRes& w, Finished& f);
+  static size_t rounded(long long bytes) { return ((size_t)bytes + 255) & ~(size_t)255; }
+  // first free range holding `bytes`, or freeRanges.end()
+  std::map<size_t, size_t>::iterator find_fit(long long bytes) {
+    for (auto it = freeRanges.begin(); it != freeRanges.end(); ++it)
+      if (it->second >= rounded(bytes)) return it;
+    return freeRanges.end();
+  }
+  size_t take(std::map<size_t, size_t>::iterator it, long long bytes) {
+    const size_t off = it->first, sz = it->second, need = rounded(bytes);
+    freeRanges.erase(it);
+    if (sz > need) freeRanges[off + need] = sz - need;
+    return off;
+  }
+  void give(size_t off, long long bytes) {
+    size_t sz = rounded(bytes);
+    auto next = freeRanges.lower_bound(off);
+    if (next != freeRanges.end() && off + sz == next->first) {
+      sz += next->second;
+      next = freeRanges.erase(next);
+    }
+    if (next != freeRanges.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += sz;
+        return;
+      }
+    }
+    freeRanges[off] = sz;
+  }
 };
 
-void t3x_server::run_one(const Pending& p, Finished& f) {
+void t3x_server::run_one(const Pending& p, const WorkerRes& w, Finished& f) {
   const t3x_request& r = p.req;
   const int ny = r.nely, nx = r.nelx, nz = r.nelz;
   std::vector<uint8_t> vox((size_t)ny * nx * nz, 1);
@@ -74,8 +121,13 @@ void t3x_server::run_one(const Pending& p, Finished& f) {
   for (size_t i = 0; i < loadN.size(); ++i)
     for (int c = 0; c < 3; ++c) loadV[3 * i + c] = d[c] * r.loadMag / (double)loadN.size();
   t3x_problem* h = nullptr;
-  int rc = t3x_create(vox.data(), ny, nx, nz, (int)fixN.size(), fixN.data(), fixS.data(), (int)loadN.size(),
-                      loadN.data(), loadV.data(), &cfg.params, &h);
+  t3x::PoolSlot slot;
+  slot.base = pool + p.offset;
+  slot.size = rounded(p.bytes);
+  slot.stream = w.stream;
+  slot.hostBuf = w.hostBuf;
+  int rc = t3x::create_pooled(vox.data(), ny, nx, nz, (int)fixN.size(), fixN.data(), fixS.data(), (int)loadN.size(),
+                              loadN.data(), loadV.data(), &cfg.params, slot, &h);
   f.c.tag = r.tag;
   f.c.footprint = p.bytes;
   if (rc == T3X_OK) {
@@ -104,28 +156,38 @@ void t3x_server::run_one(const Pending& p, Finished& f) {
 }
 
 void t3x_server::worker() {
+  WorkerRes w;
+  bool ok = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking) == hipSuccess &&
+            hipHostMalloc((void**)&w.hostBuf, 256 * sizeof(double), hipHostMallocDefault) == hipSuccess;
   for (;;) {
     Pending p;
     {
       std::unique_lock<std::mutex> lk(m);
-      // arrival order: the head request waits until its footprint fits
+      // arrival order: the head request waits until a free range of the pool
+      // holds its footprint
       cvWork.wait(lk, [&] {
-        return (stopping && queue.empty()) ||
-               (!queue.empty() && memInUse + queue.front().bytes <= cfg.memBudget);
+        return (stopping && queue.empty()) || (!queue.empty() && find_fit(queue.front().bytes) != freeRanges.end());
       });
-      if (queue.empty()) return;
+      if (queue.empty()) break;
       p = queue.front();
       queue.pop_front();
+      p.offset = take(find_fit(p.bytes), p.bytes);
       memInUse += p.bytes;
       ++running;
     }
     Finished f;
     f.c.tSubmit = p.tSubmit;
     f.c.tStart = now();
-    run_one(p, f);
+    if (ok) {
+      run_one(p, w, f);
+    } else {
+      f.c.tag = p.req.tag;
+      f.c.status = T3X_ERR_HIP;
+    }
     f.c.tEnd = now();
     {
       std::lock_guard<std::mutex> lk(m);
+      give(p.offset, p.bytes);
       memInUse -= p.bytes;
       --running;
       const long long tag = f.c.tag;
@@ -137,6 +199,8 @@ void t3x_server::worker() {
     cvWork.notify_all();
     cvDone.notify_all();
   }
+  if (w.stream) (void)hipStreamDestroy(w.stream);
+  if (w.hostBuf) (void)hipHostFree(w.hostBuf);
 }
 
 extern "C" {
@@ -156,6 +220,12 @@ int t3x_server_create(const t3x_server_config* cfg, t3x_server** out) {
     }
     s->cfg.memBudget = (long long)(0.9 * (double)freeB);
   }
+  s->cfg.memBudget &= ~255LL;
+  if (hipMalloc((void**)&s->pool, (size_t)s->cfg.memBudget) != hipSuccess) {
+    delete s;
+    return T3X_ERR_HIP;
+  }
+  s->freeRanges[0] = (size_t)s->cfg.memBudget;
   s->t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < cfg->maxActive; ++i) s->workers.emplace_back([s] { s->worker(); });
   *out = s;
@@ -169,7 +239,7 @@ int t3x_server_submit(t3x_server* s, const t3x_request* req) {
   long long bytes = 0;
   int rc = t3x_footprint(nullptr, req->nely, req->nelx, req->nelz, &s->cfg.params, req->mode, &bytes);
   if (rc != T3X_OK) return rc;
-  if (bytes > s->cfg.memBudget) return T3X_ERR_INPUT;
+  if ((long long)t3x_server::rounded(bytes) > s->cfg.memBudget) return T3X_ERR_INPUT;
   {
     std::lock_guard<std::mutex> lk(s->m);
     if (s->stopping) return T3X_ERR_INPUT;
@@ -226,6 +296,7 @@ void t3x_server_destroy(t3x_server* s) {
   }
   s->cvWork.notify_all();
   for (auto& t : s->workers) t.join();
+  if (s->pool) (void)hipFree(s->pool);
   delete s;
 }
 
