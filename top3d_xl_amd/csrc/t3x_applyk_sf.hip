@@ -44,7 +44,8 @@ namespace {
 constexpr int SF_NT = 15;          // node columns per tile edge
 constexpr int SF_CT = SF_NT + 1;   // cells per tile edge
 constexpr int SF_HT = SF_NT + 2;   // staged nodes per tile edge
-constexpr int SF_ZC = 40;          // node layers per work-group (at most; see sf_zc)
+constexpr int SF_ZC = 26;          // node layers per work-group (at most; see sf_zc): FP64 K*p at
+                                   // 256x512x256 0.60 ms against 0.62 (16) and 0.63 (40), interleaved runs
 constexpr int SF_ZC_SPARSE = 16;   // the same on domains with void tiles (shorter chunks: more tiles skipped)
 constexpr int SF_ZC_MIN = 4;       // fewest node layers per work-group when splitting for parallelism
 constexpr int SF_MIN_GROUPS = 2048;  // work-groups below which the z chunks are shortened
