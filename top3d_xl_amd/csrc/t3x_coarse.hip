@@ -554,6 +554,154 @@ __global__ void k_solve_scatter(int n, const long long* __restrict__ map, const 
   if (i < n) xdense[map[i]] = (TX)vb[i];
 }
 
+// ---- 3b. the recurrences without a launch per block ------------------------
+// A launch per block costs ~7.5 us of dispatch, ramp and drain against
+// ~1.5 us of data.  Here one launch per recurrence (forward after
+// k_solve_f0, backward after k_solve_b0) walks all its blocks: G work-groups
+// of 4 waves, wave w of the grid owning row w of every block.  A block's rows
+// carry their own readiness: each FP64 entry travels as two 64-bit words (32
+// bits of the value, 32 bits of tag), written with agent-scope stores
+// (coherent across the XCDs' L2s), and a group that needs block b - 1
+// (forward) or b + 1 (backward) polls those words until every entry shows the
+// tag of that block in this launch.  The row of G_b (or H_b) and the row's
+// own term for the next block are loaded right after a block is written, so
+// they arrive while the group waits; a step is then one poll, one 64-lane dot
+// product and one store.  Tags: a launch pair takes its first tag from
+// ChainState (the backward launch advances every group's word by nblk),
+// block b carries first + b + 1 forward and first + nblk - b backward, so
+// words left by an earlier launch never match.  All G = ceil(SB / 4) groups
+// must be resident together: 4 waves and SB doubles of LDS each, far below
+// the device's capacity.  Each wait is bounded: after kChainSpin polls the
+// launch sets fail, every group stops waiting (the solve is then garbage) and
+// the caller reports the error.  Measured alternatives (same operators):
+// one launch for everything with a published count per block, 7.6 us per
+// block; the same with an agent-scope release/acquire pair, which writes back
+// and invalidates the L2, 30 us per block.
+constexpr unsigned kChainSpin = 1u << 20;  // polls per wait (~1 s) before the launch gives up
+
+__device__ __forceinline__ double chain_wave_sum(double s) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+// entry i of a tagged vector: words 2i (high half of the value) and 2i + 1 (low half)
+__device__ __forceinline__ void chain_put(unsigned long long* w, int i, double v, unsigned tag) {
+  const unsigned long long d = (unsigned long long)__double_as_longlong(v);
+  __hip_atomic_store(w + 2 * i, (d & 0xFFFFFFFF00000000ull) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 2 * i + 1, (d << 32) | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// FWD: y_b = a_b - G_b y_{b-1} (a = Dinv_b v_b from k_solve_f0, overwritten
+// by y for k_solve_b0); backward: x_b = a_b - H_b x_{b+1} (a = Dinv_b^T y_b
+// from k_solve_b0), scattered to xdense as formed.  PT: block entries per
+// thread (SB <= PT * CT).
+template <typename TV, int PT, bool FWD>
+__global__ __launch_bounds__(CT) void k_solve_chain(int n, int SB, int nblk, const long long* __restrict__ map,
+                                                    TV* __restrict__ xdense, const double* __restrict__ M,
+                                                    double* __restrict__ a, unsigned long long* W, ChainState* st,
+                                                    const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  constexpr int NCH = PT * CT / 64;  // 64-column chunks of a row
+  extern __shared__ double chainNbr[];  // the neighbour block
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (CT / 64) + (threadIdx.x >> 6);
+  const bool act = row < SB;
+  const unsigned tag0 = st->tag[blockIdx.x];  // written by this group's previous backward launch
+  auto tag_of = [&](int b) { return FWD ? tag0 + b + 1 : tag0 + (nblk - b); };
+  const int step = FWD ? 1 : -1;
+  auto in_range = [&](int b) { return b >= 0 && b < nblk; };
+  double rg[NCH], ab = 0.0;
+  auto prefetch = [&](int b) {  // row of M_b and this row's own term, all loads in flight together
+    const double* m = M + ((size_t)b * SB + row) * SB;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const int j = lane + 64 * c;
+      rg[c] = j < SB ? m[j] : 0.0;
+    }
+    ab = a[(size_t)b * SB + row];
+  };
+  auto emit = [&](int b, double v) {
+    const int g = b * SB + row;
+    chain_put(W, g, v, tag_of(b));
+    if (FWD) a[g] = v;
+    else if (g < n) xdense[map[g]] = (TV)v;
+  };
+  // the first block is its own term
+  int b = FWD ? 0 : nblk - 1;
+  if (act) {
+    ab = a[(size_t)b * SB + row];
+    if (lane == 0) emit(b, ab);
+  }
+  b += step;
+  if (in_range(b) && act) prefetch(b);
+  for (; in_range(b); b += step) {
+    {  // block b - step into LDS once every entry carries its tag
+      const int nb = b - step;
+      const unsigned tag = tag_of(nb);
+      unsigned pending = 0;
+#pragma unroll
+      for (int k = 0; k < PT; ++k)
+        if (threadIdx.x + k * CT < SB) pending |= 1u << k;
+      double v[PT];
+      unsigned polls = 0;
+      while (pending) {
+        unsigned long long hi[PT], lo[PT];
+#pragma unroll
+        for (int k = 0; k < PT; ++k)
+          if (pending & (1u << k)) {
+            const int i = nb * SB + threadIdx.x + k * CT;
+            hi[k] = __hip_atomic_load(W + 2 * i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lo[k] = __hip_atomic_load(W + 2 * i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+#pragma unroll
+        for (int k = 0; k < PT; ++k)
+          if ((pending & (1u << k)) && (unsigned)hi[k] == tag && (unsigned)lo[k] == tag) {
+            v[k] = __longlong_as_double((long long)((hi[k] & 0xFFFFFFFF00000000ull) | (lo[k] >> 32)));
+            pending &= ~(1u << k);
+          }
+        if (!pending) break;
+        if (__hip_atomic_load(&st->fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (++polls > kChainSpin) {
+          __hip_atomic_store(&st->fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < PT; ++k)
+        if (threadIdx.x + k * CT < SB) chainNbr[threadIdx.x + k * CT] = (pending & (1u << k)) ? 0.0 : v[k];
+    }
+    __syncthreads();
+    if (act) {
+      // wave_dot's order over the registered row (four running sums over
+      // chunk quadruples while the whole quadruple is in the row, then the
+      // remaining chunks into the first), so the result is bitwise that of
+      // k_solve_step
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      bool quad = true;
+#pragma unroll
+      for (int q = 0; q < NCH / 4; ++q) {
+        const int j = lane + 256 * q;
+        quad = quad && j + 192 < SB;
+        if (quad) {
+          s0 += rg[4 * q] * chainNbr[j];
+          s1 += rg[4 * q + 1] * chainNbr[j + 64];
+          s2 += rg[4 * q + 2] * chainNbr[j + 128];
+          s3 += rg[4 * q + 3] * chainNbr[j + 192];
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (j + 64 * r < SB) s0 += rg[4 * q + r] * chainNbr[j + 64 * r];
+        }
+      }
+      const double s = ab - chain_wave_sum((s0 + s1) + (s2 + s3));
+      if (lane == 0) emit(b, s);
+      if (in_range(b + step)) prefetch(b + step);
+    }
+    __syncthreads();  // chainNbr free
+  }
+  if (!FWD && threadIdx.x == 0) st->tag[blockIdx.x] = tag0 + nblk;
+}
+
 // ---- host side -----------------------------------------------------------
 void CoarseSolver::plan(int n_, int bw_) {
   n = n_;
@@ -597,6 +745,19 @@ static void solve_impl(const CoarseSolver& C, hipStream_t s, const long long* ma
                        double* vb, const int* fl) {
   const int wpb = CT / 64;
   const int n = C.n, np = C.np, SB = C.SB, nblk = C.nblk;
+  if (C.one_launch()) {  // va, vb: np plain doubles, then 2 np tagged words each
+    auto fwd = SB <= 4 * CT ? k_solve_chain<TV, 4, true> : k_solve_chain<TV, kChainMaxSB / CT, true>;
+    auto bwd = SB <= 4 * CT ? k_solve_chain<TV, 4, false> : k_solve_chain<TV, kChainMaxSB / CT, false>;
+    const dim3 grid((SB + wpb - 1) / wpb);
+    hipLaunchKernelGGL(k_solve_f0<TV>, dim3((np + wpb - 1) / wpb), dim3(CT), 0, s, n, np, SB, map, rdense, Dinv, va,
+                       fl);
+    hipLaunchKernelGGL(fwd, grid, dim3(CT), SB * sizeof(double), s, n, SB, nblk, map, xdense, Gm, va,
+                       (unsigned long long*)(va + np), C.chain, fl);
+    hipLaunchKernelGGL(k_solve_b0, dim3((np + wpb - 1) / wpb), dim3(CT), 0, s, np, SB, DT, va, vb, fl);
+    hipLaunchKernelGGL(bwd, grid, dim3(CT), SB * sizeof(double), s, n, SB, nblk, map, xdense, H, vb,
+                       (unsigned long long*)(vb + np), C.chain, fl);
+    return;
+  }
   hipLaunchKernelGGL(k_solve_f0<TV>, dim3((np + wpb - 1) / wpb), dim3(CT), 0, s, n, np, SB, map, rdense, Dinv, va,
                      fl);
   for (int b = 1; b < nblk; ++b)

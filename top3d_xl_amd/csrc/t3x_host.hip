@@ -162,6 +162,10 @@ struct t3x_problem {
   char* arena = nullptr;
   size_t arenaSize = 0, arenaUsed = 0;
   bool ownStream = true, ownHostBuf = true;
+  // coarsest solve: one launch per recurrence (k_solve_chain) unless the
+  // lender runs other problems beside this one (its groups must all be
+  // resident together, which concurrent problems' copies would not ensure)
+  bool oneLaunchCoarse = true;
   OptState opt;
   GroupTimer timers[TG_COUNT];
   t3x_params P{};
@@ -221,6 +225,7 @@ struct t3x_problem {
   CoarseFactor cf;
   const CoarseFactor& fac() const { return cf; }
   double *cva = nullptr, *cvb = nullptr;
+  ChainState* chainSt = nullptr;
   bool assembled = false;
   // compact <-> dense maps (finest)
   int nNodes = 0, nCells = 0;
@@ -467,7 +472,8 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
   add(8 * np * 64);                                  // tile inverses
   add(8 * 2 * ((bw + 63) / 64 + 1) * 64 * 64);       // factorization panels
   add(8 * (np / SB) * SB * SB, 4);                   // Dinv, DT, G, H
-  add(8 * np, 2);                                    // solve vectors
+  add(8 * 3 * np, 2);                                // solve vectors (with the tagged words)
+  add((long long)sizeof(ChainState));                 // one-launch recurrences' words
   add(8LL * std::max(kMaxRedBlocks * 8, kMaxPartials) + 4096 * 4);  // reductions, scalars, flags
   add(4LL * kMaxPartials);                           // K*u tile list (irregular domains)
   add(NN / 16 + 1);                                  // node-chunk flags (irregular domains)
@@ -726,8 +732,16 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     f.G = P.alloc<double>(P.cs.blk_len());
     f.H = P.alloc<double>(P.cs.blk_len());
     f.status = P.alloc<int>(1);
-    P.cva = P.alloc<double>(P.cs.np);
-    P.cvb = P.alloc<double>(P.cs.np);
+    // np doubles, then (one launch per recurrence) 2 np tagged words each,
+    // zeroed: tags start at 1, so no word left in recycled memory can match
+    P.cva = P.alloc<double>(3 * (size_t)P.cs.np);
+    P.cvb = P.alloc<double>(3 * (size_t)P.cs.np);
+    P.chainSt = P.alloc<ChainState>(1);
+    T3X_HIP(hipMemsetAsync(P.cva, 0, 3 * (size_t)P.cs.np * sizeof(double), P.stream));
+    T3X_HIP(hipMemsetAsync(P.cvb, 0, 3 * (size_t)P.cs.np * sizeof(double), P.stream));
+    T3X_HIP(hipMemsetAsync(P.chainSt, 0, sizeof(ChainState), P.stream));
+    P.sync();
+    if (P.oneLaunchCoarse) P.cs.chain = P.chainSt;
   }
   // ---- compact maps (eleMapBack / nodMapBack order)
   for (long long n = 0; n < NN; ++n) {
@@ -924,6 +938,15 @@ void vcycle_standard64(t3x_problem& P, const double* r, double* z, double* sc, c
 }
 
 // ------------------------------------------------------------- PCG
+// The coarsest solve's failure word (read into hbuf[255] after a solve): a
+// wait of the one-launch recurrences expired, the solve is garbage.
+static void check_chain(t3x_problem& P) {
+  int failed = 0;
+  std::memcpy(&failed, P.hbuf + 255, sizeof(int));
+  if (!P.cs.chain || failed == 0) return;
+  throw HipError("coarsest solve: a wait between work-groups expired (one-launch recurrences)");
+}
+
 // Solving_PCG (600-639) with device-side scalars and convergence flag.
 int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxIT, double* resOut, bool* warned) {
   hipStream_t s = P.stream;
@@ -1009,7 +1032,9 @@ int pcg_elastic(t3x_problem& P, const double* b, double* y, double tol, int maxI
   }
   launch_pcg_finish(s, y, P.p, n3, sc, fl, P.fineAct());
   T3X_HIP(hipMemcpyAsync(P.hbuf, sc, SC_COUNT_PCG * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (P.cs.chain) T3X_HIP(hipMemcpyAsync(P.hbuf + 255, &P.cs.chain->fail, sizeof(int), hipMemcpyDeviceToHost, s));
   P.sync();
+  check_chain(P);
   if (resOut) *resOut = P.hbuf[SC_RES];
   if (warned) *warned = !done;  // 634-637
   return its;
@@ -1430,6 +1455,7 @@ static int create_impl(const uint8_t* voxels, int nely, int nelx, int nelz, int 
     P->ownStream = false;
     P->hbuf = slot->hostBuf;
     P->ownHostBuf = false;
+    P->oneLaunchCoarse = slot->oneLaunchCoarse;
   }
   P->nely = nely; P->nelx = nelx; P->nelz = nelz;
   if (!voxels || nelx < 3 || nely < 3 || nelz < 3) {  // 532-534
@@ -1791,6 +1817,11 @@ int t3x_vcycle(t3x_problem* h, const double* r, double* x) {
       else vcycle(*h, h->r, h->z, h->sc, nullptr, false);
     }
     download_nodal(*h, h->z, x);
+    if (h->cs.chain) {
+      T3X_HIP(hipMemcpyAsync(h->hbuf + 255, &h->cs.chain->fail, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+      h->sync();
+      check_chain(*h);
+    }
     return (int)T3X_OK;
   });
 }

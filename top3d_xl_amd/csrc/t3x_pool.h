@@ -18,6 +18,9 @@ struct PoolSlot {
   size_t size = 0;
   hipStream_t stream = nullptr;
   double* hostBuf = nullptr;  // 256 doubles, pinned
+  // coarsest solve in one launch per recurrence: only where no other problem
+  // can hold the device's work-group slots meanwhile (one active request)
+  bool oneLaunchCoarse = false;
 };
 
 // t3x_create with the built-in or explicit one-case boundary conditions, on
