@@ -41,9 +41,11 @@ def parse_size(s):
     return ny, nx, nz
 
 
+# mixed: the precision form ('Standard' V(1,1), omega 0.4, hierarchy deepened
+# where the coarsest band Cholesky is costly: coarsestControl 0);
 # fp64_standard: the precision form's cycle (and omega) in FP64, which
 # separates what the FP32 arithmetic gains from what the cycle gains
-VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=1, vcycle=1, omega=0.4),
+VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=1, vcycle=1, omega=0.4, coarsestControl=0),
             "mixed_adapted": dict(precision=1), "fp64_standard": dict(precision=0, vcycle=1, omega=0.4)}
 
 
@@ -261,7 +263,12 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-nloop-scale", type=float, default=1.0)
+    ap.add_argument("--coarsest-control", type=int, default=None,
+                    help="coarsestControl of the mixed-precision variants (<= 0: depth by coarsest band cost)")
     args = ap.parse_args()
+    if args.coarsest_control is not None:
+        for k in ("mixed", "mixed_adapted"):
+            VARIANTS[k] = dict(VARIANTS[k], coarsestControl=args.coarsest_control)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build measures one GPU only: the boxes here have one MI355X and the "

@@ -44,7 +44,10 @@ typedef struct t3x_params {
   double tol;            /* tol_                (30) 1e-3 */
   int maxIT;             /* maxIT_              (31) 800 */
   double omega;          /* weightFactorJacobi_ (32) 0.6 */
-  int coarsestControl;   /* coarsestResolutionControl_ (33) 50000 */
+  int coarsestControl;   /* coarsestResolutionControl_ (33) 50000; <= 0: the
+                            count for 50000, deepened while the coarsest band
+                            Cholesky would cost more than 1e11 flop (an option
+                            of this library) */
   int precision;         /* 0 = FP64 plain path;
                             1 = mixed: FP64 CG (K*p, vectors, dots, step
                                 lengths) preconditioned by the V-cycle with FP32

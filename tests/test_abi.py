@@ -67,6 +67,19 @@ def test_footprint_estimate_scales():
     assert t3x.footprint(64, 128, 64, precision=1) > b  # FP32 V-cycle copies
 
 
+def test_auto_depth_only_where_the_coarsest_band_is_costly():
+    """coarsestControl <= 0 deepens the hierarchy while the coarsest band
+    Cholesky would cost more than 1e11 flop: at 192x384x192 (coarsest 48x24x24
+    cells, 91,875 DOF, bandwidth 1,953) one level more halves the footprint
+    (the solve-block operators dominate it); at 256x512x256 (28,611 DOF,
+    bandwidth 921) the hierarchy, and so the footprint, is unchanged."""
+    import top3d_xl_amd as t3x
+    form = dict(precision=1, vcycle=1, omega=0.4)
+    ref192, auto192 = t3x.footprint(192, 384, 192, **form), t3x.footprint(192, 384, 192, coarsestControl=0, **form)
+    assert auto192 < 0.6 * ref192
+    assert t3x.footprint(256, 512, 256, coarsestControl=0, **form) == t3x.footprint(256, 512, 256, **form)
+
+
 def test_create_multi_checks_arguments_before_device():
     """t3x_create_multi refuses null or negative boundary-condition arrays with
     T3X_ERR_INPUT before it looks for a device."""

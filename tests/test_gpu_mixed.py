@@ -211,6 +211,30 @@ def test_mixed_design_loop_first_iterations(prec, case):
     assert np.all(errs <= 1e-4)
 
 
+def test_auto_depth_hierarchy_192():
+    """coarsestControl <= 0 at 192x384x192: one level more than the
+    reference's count (4 against 3; coarsest 12,675 DOF instead of 91,875),
+    and the precision form on it solves the solid and the first design to the
+    bar against the FP64 path on the reference's hierarchy (no oracle run at
+    this size: the FP64 GPU path is the oracle-checked one)."""
+    import top3d_xl_amd as t3x
+    vox = np.ones((192, 384, 192), bool)
+    print("\n[gpu-mixed] auto depth 192x384x192", flush=True)
+    f = t3x.Problem(vox)
+    assert f.num_levels == 3
+    rf = f.run("GLOBAL", 0.12, 1, math.sqrt(3))
+    f.close()
+    g = t3x.Problem(vox, coarsestControl=0, **PRECISIONS["mixed_std"])
+    assert g.num_levels == 4
+    assert (g.levels[-1]["resX"], g.levels[-1]["resY"], g.levels[-1]["resZ"]) == (24, 12, 12)
+    rg = g.run("GLOBAL", 0.12, 1, math.sqrt(3))
+    g.close()
+    ec = abs(rg["cSolid"] - rf["cSolid"]) / rf["cSolid"]
+    e1 = abs(rg["cHist"][0] - rf["cHist"][0]) / rf["cHist"][0]
+    print(f"  cSolid rel err {ec:.2e}, first design {e1:.2e}; its fp64 {rf['lssIts'].tolist()} auto {rg['lssIts'].tolist()}")
+    assert ec <= C_TOL and e1 <= C_TOL
+
+
 def test_standard_vcycle_matches_numpy():
     """The FP32 'Standard' cycle against a float64 numpy V(1,1) on the brute-force
     Galerkin hierarchy (tests/bruteforce.py; independent of the oracle)."""

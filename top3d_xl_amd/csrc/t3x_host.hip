@@ -339,13 +339,36 @@ namespace {
 struct LevelDims {
   int ny, nx, nz, span;
 };
+// coarsestControl <= 0 (not the reference's: an option of this library):
+// the reference's count for 50000, then one level more while the coarsest
+// box's band Cholesky, n bw^2 with n = 3 nodes and bw = 3 (1 + c0 + c0 c1)
+// over its sorted node extents c0 <= c1, exceeds kAutoCoarseCost and the
+// dimensions divide (the fine grid keeps its padding).  Cuboid
+// 256x512x256: 2.4e10, unchanged; 192x384x192: 3.5e11 -> 3.8e9.
+constexpr double kAutoCoarseCost = 1e11;
 std::vector<LevelDims> level_dims(int nely, int nelx, int nelz, long long nv, int coarsestControl) {
+  const bool autoDepth = coarsestControl <= 0;
+  if (autoDepth) coarsestControl = 50000;
   int cnt = 0;
   while (nv >= coarsestControl) {
     ++cnt;
     nv = (long long)std::llround((double)nv / 8.0);
   }
   cnt = std::max(3, cnt);
+  if (autoDepth) {
+    auto band_cost = [&](int c) {  // coarsest box of cnt = c: cells fine / 2^c per axis
+      const int pw = 1 << c;
+      long long e[3] = {(nely + pw - 1) / pw + 1LL, (nelx + pw - 1) / pw + 1LL, (nelz + pw - 1) / pw + 1LL};
+      std::sort(e, e + 3);
+      const double n = 3.0 * e[0] * e[1] * e[2], bw = 3.0 * (1 + e[0] + e[0] * e[1]);
+      return n * bw * bw;
+    };
+    auto divides = [&](int c) {
+      const int pw = 1 << c;
+      return nely % pw == 0 && nelx % pw == 0 && nelz % pw == 0 && std::min({nely, nelx, nelz}) / pw >= 2;
+    };
+    while (band_cost(cnt) > kAutoCoarseCost && divides(cnt) && divides(cnt + 1)) ++cnt;
+  }
   const int pw = 1 << cnt;
   auto padded = [&](int n) { return (n + pw - 1) / pw * pw; };
   // (cnt + 1) levels, then one merged by the span-4 first coarsening
