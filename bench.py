@@ -41,12 +41,15 @@ def parse_size(s):
     return ny, nx, nz
 
 
-# mixed: the precision form ('Standard' V(1,1), omega 0.4, hierarchy deepened
-# where the coarsest band Cholesky is costly: coarsestControl 0);
+# mixed: the precision form ('Standard' V(1,1), omega 0.4, FP16 coarse
+# Galerkin matrices with per-cell scales, hierarchy deepened where the
+# coarsest band Cholesky is costly: coarsestControl 0);
+# mixed_fp32_coarse: the same with FP32 coarse matrices (what FP16 storage gains);
 # fp64_standard: the precision form's cycle (and omega) in FP64, which
 # separates what the FP32 arithmetic gains from what the cycle gains
-VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=1, vcycle=1, omega=0.4, coarsestControl=0),
-            "mixed_adapted": dict(precision=1), "fp64_standard": dict(precision=0, vcycle=1, omega=0.4)}
+VARIANTS = {"fp64": dict(precision=0), "mixed": dict(precision=2, vcycle=1, omega=0.4, coarsestControl=0),
+            "mixed_adapted": dict(precision=1), "fp64_standard": dict(precision=0, vcycle=1, omega=0.4),
+            "mixed_fp32_coarse": dict(precision=1, vcycle=1, omega=0.4, coarsestControl=0)}
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_fp64_256x512x256.md")
@@ -185,7 +188,8 @@ def serving_leg(args):
     return {"requests": len(reqs), "rate_per_s": args.serve_rate, "nloop_scale": args.serve_nloop_scale,
             "sizes_s": sorted({r["shape"][0] for r in reqs}),
             "modes": {m: sum(1 for r in reqs if r["mode"] == m) for m in ("GLOBAL", "LOCAL")},
-            "engine": dict(max_active=4, precision="mixed 'Standard' (omega 0.4)", **summarize(reqs, done, wall)),
+            "engine": dict(max_active=4, precision="precision form (mixed 'Standard', omega 0.4, FP16 coarse "
+                                                    "matrices, coarsestControl 0)", **summarize(reqs, done, wall)),
             "one_at_a_time_fp64": summarize(reqs, done1, wall1)}
 
 
@@ -267,7 +271,7 @@ def main():
                     help="coarsestControl of the mixed-precision variants (<= 0: depth by coarsest band cost)")
     args = ap.parse_args()
     if args.coarsest_control is not None:
-        for k in ("mixed", "mixed_adapted"):
+        for k in ("mixed", "mixed_adapted", "mixed_fp32_coarse"):
             VARIANTS[k] = dict(VARIANTS[k], coarsestControl=args.coarsest_control)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -280,7 +284,7 @@ def main():
     model = make_model(args)
     runs = {}
     names = {"fp64": ["fp64"], "mixed": ["mixed"],
-             "both": ["fp64", "mixed", "mixed_adapted", "fp64_standard"]}[args.precision]
+             "both": ["fp64", "mixed", "mixed_adapted", "fp64_standard", "mixed_fp32_coarse"]}[args.precision]
     for name in names:
         print(f"[bench] {name}: {VARIANTS[name]}", file=sys.stderr, flush=True)
         runs[name] = gpu_run(args, VARIANTS[name], model, sample_designs=(name == "mixed" and not args.no_accuracy))
@@ -342,8 +346,9 @@ def main():
                     f"{g['nE']} solid voxels), {3 * g['nN']} DOF, V0=0.4, rMin=sqrt(3), bottom-slab fixed, "
                     f"unit load on top-slab surface nodes"),
                    "precision": head if head == "fp64" else
-                   "mixed: FP64 PCG (K*p, vectors, dots) + FP32 'Standard' V(1,1) cycle (omega 0.4) "
-                   "around the FP64 coarsest Cholesky"},
+                   "mixed: FP64 PCG (K*p, vectors, dots) + FP32 'Standard' V(1,1) cycle (omega 0.4; "
+                   "coarse Galerkin matrices stored FP16 with per-cell power-of-two scales) around the FP64 "
+                   "coarsest Cholesky"},
         "pcg_dof_per_s": round(g["pcg_dof_per_s"], 1),
         "pcg_iterations": g["pcg_its"],
         "pcg_ms_per_iteration": round(1e3 * g["cg_s"] / max(g["pcg_its"], 1), 4),

@@ -52,7 +52,10 @@ typedef struct t3x_params {
                             1 = mixed: FP64 CG (K*p, vectors, dots, step
                                 lengths) preconditioned by the V-cycle with FP32
                                 level vectors, diagonals and smoother output
-                                around the FP64 coarsest Cholesky */
+                                around the FP64 coarsest Cholesky;
+                            2 = as 1, with the 'Standard' cycle's coarse-level
+                                Galerkin matrices stored in FP16 with a
+                                power-of-two scale per cell (FP32 products) */
   int checkEvery;        /* PCG convergence flag read-back interval (>=1) */
   int timing;            /* 1 = time kernel groups with HIP events (t3x_kernel_time) */
   int vcycle;            /* typeVcycle_ (34): 0 = 'Adapted' (747-781); 1 = 'Standard'

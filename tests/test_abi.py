@@ -65,6 +65,8 @@ def test_footprint_estimate_scales():
     assert 6 * a < b < 10 * a  # ~8x the nodes
     assert t3x.footprint(64, 128, 64, mode="LOCAL") > b  # MMA state
     assert t3x.footprint(64, 128, 64, precision=1) > b  # FP32 V-cycle copies
+    std = dict(vcycle=1, omega=0.4)
+    assert t3x.footprint(64, 128, 64, precision=2, **std) < t3x.footprint(64, 128, 64, precision=1, **std)  # FP16 Ks
 
 
 def test_auto_depth_only_where_the_coarsest_band_is_costly():

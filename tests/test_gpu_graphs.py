@@ -14,7 +14,8 @@ from tests.test_multiload import model as two_case_model
 pytestmark = pytest.mark.gpu
 
 FORMS = {"fp64": dict(precision=0), "fp64_standard": dict(precision=0, vcycle=1),
-         "mixed": dict(precision=1, vcycle=1, omega=0.4), "mixed_adapted": dict(precision=1)}
+         "mixed": dict(precision=1, vcycle=1, omega=0.4), "mixed_adapted": dict(precision=1),
+         "mixed_f16": dict(precision=2, vcycle=1, omega=0.4)}
 
 
 @pytest.mark.parametrize("form", list(FORMS))

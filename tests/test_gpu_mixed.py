@@ -50,7 +50,8 @@ C_TOL = 1e-6
 # the mixed form with the reference's 'Adapted' cycle, and with the 'Standard'
 # V(1,1) cycle (759-777) at omega = 0.4: with the reference's 0.6 the damped
 # Jacobi smoother diverged on high-contrast designs (CPU experiment, 24x48x24)
-PRECISIONS = {"mixed": dict(precision=1), "mixed_std": dict(precision=1, vcycle=1, omega=0.4)}
+PRECISIONS = {"mixed": dict(precision=1), "mixed_std": dict(precision=1, vcycle=1, omega=0.4),
+              "mixed_std_f16": dict(precision=2, vcycle=1, omega=0.4)}
 
 
 @pytest.fixture(scope="module", params=[(c, p) for p in PRECISIONS for c in CASES], ids=lambda v: f"{v[0]}-{v[1]}")
@@ -92,7 +93,12 @@ def test_mixed_vcycle(mpair):
     # FP32 level vectors around an FP64 coarse solve; the 'Standard' form also
     # forms FP32 defects r - K x on every level
     g, o = mpair["g"], mpair["o"]
-    std = mpair["prec"] == "mixed_std"
+    std = mpair["prec"] in ("mixed_std", "mixed_std_f16")
+    # FP16 coarse matrices (precision 2): entries rounded to 11 significant
+    # bits below their cell's power-of-two scale, so the cycle is a nearby
+    # preconditioner, held to 5e-3; the solves' accuracy is what
+    # test_mixed_pcg_compliance holds to the 1e-6 bar
+    bar = {"mixed": 2e-5, "mixed_std": 1e-4, "mixed_std_f16": 5e-3}[mpair["prec"]]
     if std:
         o.set_vcycle("Standard")
         o.set_omega(PRECISIONS["mixed_std"]["omega"])
@@ -103,7 +109,7 @@ def test_mixed_vcycle(mpair):
             r = rng.standard_normal(3 * o.nN)
             d = rel(g.vcycle(r), o.vcycle(r))
             print(f"  vcycle rel {d:.2e}")
-            assert d < (1e-4 if std else 2e-5)
+            assert d < bar
     finally:
         o.set_vcycle("Adapted")
         o.set_omega(0.6)

@@ -24,7 +24,8 @@ def test_footprint_bounds_allocation():
     std = dict(precision=1, vcycle=1, omega=0.4)
     cases = [(np.ones((16, 32, 16), bool), None, "GLOBAL", {}), (np.ones((24, 48, 24), bool), None, "LOCAL", {}),
              (np.ones((24, 48, 24), bool), None, "GLOBAL", dict(precision=1)),
-             (np.ones((24, 48, 24), bool), None, "GLOBAL", std), (g["voxels"], g, "GLOBAL", {})]
+             (np.ones((24, 48, 24), bool), None, "GLOBAL", std),
+             (np.ones((24, 48, 24), bool), None, "GLOBAL", dict(std, precision=2)), (g["voxels"], g, "GLOBAL", {})]
     for vox, bc, mode, prec in cases:
         if bc is None:
             P = t3x.Problem(vox, **prec)
@@ -92,7 +93,7 @@ def test_alone_equals_mix():
           f"peak admitted {peak / 2**20:.0f} MiB of budget {budget / 2**20:.0f} MiB")
 
 
-PRECISION_FORM = dict(precision=1, vcycle=1, omega=0.4)
+PRECISION_FORM = dict(precision=2, vcycle=1, omega=0.4, coarsestControl=0)  # the shipped precision form
 
 
 @pytest.mark.parametrize("s,nloop,params", [(16, 5, {}), (48, 30, PRECISION_FORM)], ids=["fp64-16", "mixed-48"])

@@ -2,6 +2,8 @@
 // element matrix in sum-factorised form, is in t3x_applyk_sf.hip.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "t3x_kernels.h"
 #include "t3x_reduce.h"
 
@@ -24,11 +26,14 @@ enum { EPI_RESID = 1, EPI_SMOOTH = 2 };
 // Coarse levels carry no fixed DOFs (the boundary conditions are inside the
 // Galerkin matrices); vectors and matrices are zero on void nodes and cells.
 // T = float (the mixed form's FP32 smoother) or double (the FP64 'Standard'
-// cycle).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const T* __restrict__ Ks, const T* __restrict__ u,
+// cycle); TK, the matrices' storage: T, or FP16 with a per-cell power-of-two
+// scale (the products formed unscaled, the cell's outputs scaled once,
+// exactly).
+template <typename T, typename TK = T>
+__global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const TK* __restrict__ Ks, const T* __restrict__ u,
                                                       T* __restrict__ Yc, const int* fl,
-                                                      const uint8_t* __restrict__ emask) {
+                                                      const uint8_t* __restrict__ emask,
+                                                      const float* __restrict__ scale = nullptr) {
   if (fl && fl[FL_DONE]) return;
   const int plane = g.NY * g.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
@@ -57,13 +62,34 @@ __global__ __launch_bounds__(kBlock) void k_cell_Ks_u(Geo g, const T* __restrict
   for (int q = 0; q < 24; ++q) {
 #pragma unroll
     for (int p = 0; p <= q; ++p) {
-      const T v = Ks[(long long)(q * (q + 1) / 2 + p) * NN + cell];
+      const T v = T(Ks[(long long)(q * (q + 1) / 2 + p) * NN + cell]);
       ye[p] += v * ue[q];
       if (p != q) ye[q] += v * ue[p];
     }
   }
+  if constexpr (std::is_same<TK, T>::value) {
 #pragma unroll
-  for (int i = 0; i < 24; ++i) Yc[(long long)i * NN + cell] = ye[i];
+    for (int i = 0; i < 24; ++i) Yc[(long long)i * NN + cell] = ye[i];
+  } else {
+    const T sc = T(scale[cell]);
+#pragma unroll
+    for (int i = 0; i < 24; ++i) Yc[(long long)i * NN + cell] = ye[i] * sc;
+  }
+}
+
+// Ks -> FP16 with a per-cell power-of-two scale (one thread per cell)
+__global__ __launch_bounds__(kBlock) void k_scale_ks_f16(Geo g, const double* __restrict__ Ks,
+                                                         __half* __restrict__ Ksh, float* __restrict__ scale) {
+  const long long NN = g.NN;
+  const long long cell = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (cell >= NN) return;
+  double m = 0.0;
+  for (int e = 0; e < kTri; ++e) m = fmax(m, fabs(Ks[(long long)e * NN + cell]));
+  int ex = 0;
+  if (m > 0.0) (void)frexp(m, &ex);  // m < 2^ex
+  const double s = m > 0.0 ? ldexp(1.0, ex) : 1.0;
+  scale[cell] = (float)s;
+  for (int e = 0; e < kTri; ++e) Ksh[(long long)e * NN + cell] = __float2half((float)(Ks[(long long)e * NN + cell] / s));
 }
 
 template <typename T, int EPI>
@@ -110,18 +136,19 @@ static dim3 node_grid(const Geo& g) { return dim3((g.NY * g.NX + kBlock - 1) / k
 
 long long coarse_ks_tmp_len(const Geo& g) { return 24LL * g.NN; }
 
-template <typename T>
-static void coarse_resid(hipStream_t s, const Geo& g, const T* Ks, const T* x, const T* r, T* d, T* tmp,
-                         const int* fl, const uint8_t* emask, const uint8_t* nmask) {
-  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl, emask);
+template <typename T, typename TK>
+static void coarse_resid(hipStream_t s, const Geo& g, const TK* Ks, const T* x, const T* r, T* d, T* tmp,
+                         const int* fl, const uint8_t* emask, const uint8_t* nmask, const float* scale = nullptr) {
+  hipLaunchKernelGGL((k_cell_Ks_u<T, TK>), node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl, emask, scale);
   hipLaunchKernelGGL((k_node_gather_Ks<T, EPI_RESID>), node_grid(g), dim3(kBlock), 0, s, g, (const T*)tmp, x, d, r,
                      (const T*)nullptr, 0.0, fl, nmask);
 }
 
-template <typename T>
-static void coarse_smooth(hipStream_t s, const Geo& g, const T* Ks, const T* x, const T* r, const T* diag,
-                          double omega, T* out, T* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask) {
-  hipLaunchKernelGGL(k_cell_Ks_u<T>, node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl, emask);
+template <typename T, typename TK>
+static void coarse_smooth(hipStream_t s, const Geo& g, const TK* Ks, const T* x, const T* r, const T* diag,
+                          double omega, T* out, T* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask,
+                          const float* scale = nullptr) {
+  hipLaunchKernelGGL((k_cell_Ks_u<T, TK>), node_grid(g), dim3(kBlock), 0, s, g, Ks, x, tmp, fl, emask, scale);
   hipLaunchKernelGGL((k_node_gather_Ks<T, EPI_SMOOTH>), node_grid(g), dim3(kBlock), 0, s, g, (const T*)tmp, x, out,
                      r, diag, omega, fl, nmask);
 }
@@ -134,6 +161,20 @@ void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const 
                             const float* diag, double omega, float* out, float* tmp, const int* fl,
                             const uint8_t* emask, const uint8_t* nmask) {
   coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl, emask, nmask);
+}
+void launch_coarse_resid32(hipStream_t s, const Geo& g, const __half* Ks, const float* scale, const float* x,
+                           const float* r, float* d, float* tmp, const int* fl, const uint8_t* emask,
+                           const uint8_t* nmask) {
+  coarse_resid(s, g, Ks, x, r, d, tmp, fl, emask, nmask, scale);
+}
+void launch_coarse_smooth32(hipStream_t s, const Geo& g, const __half* Ks, const float* scale, const float* x,
+                            const float* r, const float* diag, double omega, float* out, float* tmp, const int* fl,
+                            const uint8_t* emask, const uint8_t* nmask) {
+  coarse_smooth(s, g, Ks, x, r, diag, omega, out, tmp, fl, emask, nmask, scale);
+}
+void launch_scale_ks_f16(hipStream_t s, const Geo& g, const double* Ks, __half* Ksh, float* scale) {
+  hipLaunchKernelGGL(k_scale_ks_f16, dim3((unsigned)((g.NN + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, g, Ks, Ksh,
+                     scale);
 }
 void launch_coarse_resid64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
                            double* d, double* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask) {

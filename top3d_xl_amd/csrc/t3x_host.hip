@@ -318,7 +318,8 @@ struct t3x_problem {
   double* xTilde() { return ca; }   // LOCAL-mode state between iterations
   double* volMax() { return cb; }
   long long NN() const { return lv[0].NN; }
-  bool mixed() const { return P.precision == 1; }  // FP32 V-cycle
+  bool mixed() const { return P.precision >= 1; }  // FP32 V-cycle
+  bool f16() const { return P.precision == 2; }    // ... with FP16 coarse Galerkin copies
   const uint8_t* design_mask() const { return active ? active : lv[0].emask; }
   bool standard() const { return P.vcycle == 1; }  // 'Standard' V-cycle (FP64 or, mixed, FP32)
   void sync() { T3X_HIP(hipStreamSynchronize(stream)); }
@@ -447,7 +448,7 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
                              int nCases = 1) {
   const std::vector<LevelDims> d = level_dims(nely, nelx, nelz, nv, prm.coarsestControl);
   const int L = (int)d.size();
-  const bool mixed = prm.precision == 1;
+  const bool mixed = prm.precision >= 1;
   auto NNof = [&](int l) { return (long long)(d[l].ny + 1) * (d[l].nx + 1) * (d[l].nz + 1); };
   long long b = 0, nalloc = 0;
   auto add = [&](long long bytes, int count = 1) { b += bytes * count; nalloc += count; };
@@ -461,7 +462,8 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
     if (mixed) {
       if (l < L - 1) add(12 * NN);                   // diagf
       if (l >= 1) add(12 * NN, 2);                   // rf, xf
-      if (prm.vcycle == 1 && l >= 1 && l < L - 1) add(12 * NN), add(4LL * kTri * NN);  // tf, Ksf
+      if (prm.vcycle == 1 && l >= 1 && l < L - 1)  // tf, Ksf (or Ksh and its scales)
+        add(12 * NN), add((prm.precision == 2 ? 2LL : 4LL) * kTri * NN + (prm.precision == 2 ? 4 * NN : 0));
     }
     if (l + 1 < L) tmp = std::max(tmp, 3LL * (d[l + 1].ny + 1) * (d[l].nx + 1) * (d[l].nz + 1));
     if (prm.vcycle == 1 && l >= 1 && l < L - 1) tmp = std::max(tmp, (mixed ? 12 : 24) * NN);  // coarse smoother products
@@ -638,7 +640,12 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       }
       if (P.standard() && l >= 1 && l < P.L - 1) {
         V.tf = P.alloc<float>(3 * V.NN);
-        V.Ksf = P.alloc<float>((size_t)kTri * V.NN);
+        if (P.f16()) {
+          V.Ksh = P.alloc<__half>((size_t)kTri * V.NN);
+          V.Kscale = P.alloc<float>(V.NN);
+        } else {
+          V.Ksf = P.alloc<float>((size_t)kTri * V.NN);
+        }
       }
     }
   }
@@ -812,7 +819,11 @@ int assemble(t3x_problem& P) {
     for (int l = 0; l < P.L - 1; ++l) launch_convert(s, P.lv[l].diagf, P.lv[l].diag, 3 * P.lv[l].NN);
   if (P.standard() && P.mixed()) {
     launch_convert(s, P.E32, P.E, P.NN());
-    for (int l = 1; l < P.L - 1; ++l) launch_convert(s, P.lv[l].Ksf, P.lv[l].Ks, (long long)kTri * P.lv[l].NN);
+    for (int l = 1; l < P.L - 1; ++l) {
+      Level& V = P.lv[l];
+      if (V.Ksh) launch_scale_ks_f16(s, geo(V), V.Ks, V.Ksh, V.Kscale);
+      else launch_convert(s, V.Ksf, V.Ks, (long long)kTri * V.NN);
+    }
   }
   const Level& C = P.lv[P.L - 1];
   auto& f = P.cf;
@@ -901,7 +912,8 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   for (int l = 1; l < L - 1; ++l) {
     Level& V = P.lv[l];
     launch_jacobi0(s, (const float*)V.rf, V.diagf, V.xf, om, 3 * V.NN, fl);
-    launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, tmp, fl, V.emask, V.nmask);
+    if (V.Ksh) launch_coarse_resid32(s, geo(V), V.Ksh, V.Kscale, V.xf, V.rf, V.tf, tmp, fl, V.emask, V.nmask);
+    else launch_coarse_resid32(s, geo(V), V.Ksf, V.xf, V.rf, V.tf, tmp, fl, V.emask, V.nmask);
     launch_restrict(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, (const float*)V.tf, tmp, P.lv[l + 1].nmask,
                     P.lv[l + 1].rf, fl);
   }
@@ -912,7 +924,10 @@ void vcycle_standard(t3x_problem& P, const double* r, float* z, double* sc, cons
   for (int l = L - 2; l >= 1; --l) {
     Level& V = P.lv[l];
     launch_prolong_add(s, geo(V), geo(P.lv[l + 1]), P.lv[l + 1].span, P.lv[l + 1].xf, V.xf, V.nmask, fl);  // 770
-    launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, tmp, fl, V.emask, V.nmask);  // 775
+    if (V.Ksh)
+      launch_coarse_smooth32(s, geo(V), V.Ksh, V.Kscale, V.xf, V.rf, V.diagf, om, V.tf, tmp, fl, V.emask, V.nmask);
+    else
+      launch_coarse_smooth32(s, geo(V), V.Ksf, V.xf, V.rf, V.diagf, om, V.tf, tmp, fl, V.emask, V.nmask);  // 775
     std::swap(V.xf, V.tf);
   }
   launch_prolong_add(s, g0, geo(P.lv[1]), P.lv[1].span, P.lv[1].xf, P.x32, P.lv[0].nmask, fl);
@@ -1486,8 +1501,9 @@ static int create_impl(const uint8_t* voxels, int nely, int nelx, int nelz, int 
     *out = P.release();
     return T3X_ERR_INPUT;
   }
-  if ((P->P.precision != 0 && P->P.precision != 1) || (P->P.vcycle != 0 && P->P.vcycle != 1)) {
-    P->err = "params: precision must be 0 (FP64) or 1 (mixed); vcycle 0 ('Adapted') or 1 ('Standard')";
+  if (P->P.precision < 0 || P->P.precision > 2 || (P->P.vcycle != 0 && P->P.vcycle != 1)) {
+    P->err = "params: precision must be 0 (FP64), 1 (mixed) or 2 (mixed, FP16 coarse matrices); vcycle 0 "
+             "('Adapted') or 1 ('Standard')";
     *out = P.release();
     return T3X_ERR_INPUT;
   }

@@ -14,6 +14,7 @@
 //     mask 0; vectors are kept exactly 0 there.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -59,6 +60,8 @@ struct Level {
   float* tf = nullptr;       // FP32 'Standard' cycle: defect / smoother output (levels 1..L-2)
   float* diagf = nullptr;
   float* Ksf = nullptr;      // FP32 copy of Ks (levels 1..L-2, 'Standard' cycle)
+  __half* Ksh = nullptr;     // FP16 copy instead (precision 2): Ks / Kscale per cell
+  float* Kscale = nullptr;   // its per-cell power-of-two scales
   int* sfTiles = nullptr;      // finest level: K*u tiles with a solid cell (null: all)
   int nSfTiles = 0;
 };

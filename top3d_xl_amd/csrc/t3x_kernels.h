@@ -144,6 +144,16 @@ void launch_coarse_resid32(hipStream_t s, const Geo& g, const float* Ks, const f
 void launch_coarse_smooth32(hipStream_t s, const Geo& g, const float* Ks, const float* x, const float* r,
                             const float* diag, double omega, float* out, float* tmp, const int* fl,
                             const uint8_t* emask, const uint8_t* nmask);
+// FP16 matrices with per-cell power-of-two scales (the products formed in FP32)
+void launch_coarse_resid32(hipStream_t s, const Geo& g, const __half* Ks, const float* scale, const float* x,
+                           const float* r, float* d, float* tmp, const int* fl, const uint8_t* emask,
+                           const uint8_t* nmask);
+void launch_coarse_smooth32(hipStream_t s, const Geo& g, const __half* Ks, const float* scale, const float* x,
+                            const float* r, const float* diag, double omega, float* out, float* tmp, const int* fl,
+                            const uint8_t* emask, const uint8_t* nmask);
+// Ks (kTri x NN, FP64) -> FP16 entries Ks / scale, scale the power of two at
+// or above the cell's largest |entry| (1 for a zero cell)
+void launch_scale_ks_f16(hipStream_t s, const Geo& g, const double* Ks, __half* Ksh, float* scale);
 void launch_coarse_resid64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,
                            double* d, double* tmp, const int* fl, const uint8_t* emask, const uint8_t* nmask);
 void launch_coarse_smooth64(hipStream_t s, const Geo& g, const double* Ks, const double* x, const double* r,

@@ -208,7 +208,7 @@ extern "C" {
 
 int t3x_server_create(const t3x_server_config* cfg, t3x_server** out) {
   if (!cfg || !out || cfg->maxActive < 1) return T3X_ERR_INPUT;
-  if (cfg->params.precision != 0 && cfg->params.precision != 1) return T3X_ERR_INPUT;
+  if (cfg->params.precision < 0 || cfg->params.precision > 2) return T3X_ERR_INPUT;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return T3X_ERR_NO_DEVICE;
   t3x_server* s = new t3x_server();
