@@ -136,15 +136,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
     const long long cyx[4] = {Jy + (long long)gc.NY * Jx, Jy1 + (long long)gc.NY * Jx, Jy + (long long)gc.NY * Jx1,
                               Jy1 + (long long)gc.NY * Jx1};
     const int z0 = blockIdx.y * zc, z1 = min(gf.NZ, z0 + zc);
-    // the 8 coarse values of this column's cell, reloaded only when the
-    // coarse z interval changes (every S planes)
+    // the 8 coarse values of this column's cell, reloaded only when a solid
+    // node needs a coarse z interval other than the loaded one
     double cv[8][3];
     int cJz = -1;
     for (int z = z0; z < z1; ++z) {
       const long long n = idx + (long long)plane * z;
       const bool solid = nmaskF[n] != 0;
       const int Jz = z / S, tz = z - S * Jz;
-      if (Jz != cJz) {
+      if (solid && Jz != cJz) {
         cJz = Jz;
         const int Jz1 = min(Jz + 1, gc.NZ - 1);
         const long long zo[2] = {planeC * Jz, planeC * Jz1};
@@ -156,19 +156,19 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
           cv[c][2] = (double)xc[2 * NC + J];
         }
       }
-      const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
-      double p[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
-        const double w = wy[a] * wx[b] * wz[d];
-        p[0] += w * cv[c][0];
-        p[1] += w * cv[c][1];
-        p[2] += w * cv[c][2];
-      }
       // void nodes hold 0 in every level vector (all of their writers leave
-      // them 0): no load, no store
+      // them 0): no load, no store, no interpolation
       if (solid) {
+        const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
+        double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
+          const double w = wy[a] * wx[b] * wz[d];
+          p[0] += w * cv[c][0];
+          p[1] += w * cv[c][1];
+          p[2] += w * cv[c][2];
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           if (ADD) {
