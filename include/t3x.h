@@ -57,7 +57,8 @@ typedef struct t3x_params {
                                 Galerkin matrices stored in FP16 with a
                                 power-of-two scale per cell (FP32 products) */
   int checkEvery;        /* PCG convergence flag read-back interval (>=1) */
-  int timing;            /* 1 = time kernel groups with HIP events (t3x_kernel_time) */
+  int timing;            /* 1 = time kernel groups with HIP events (t3x_kernel_time);
+                            2 = only the fine K*p group (fewer event packets) */
   int vcycle;            /* typeVcycle_ (34): 0 = 'Adapted' (747-781); 1 = 'Standard'
                             (V(1,1) damped Jacobi with K*u on every level, 759-777;
                             coarse levels through the stored Galerkin matrices,
