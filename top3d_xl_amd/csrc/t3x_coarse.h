@@ -27,6 +27,10 @@ struct CoarseSolver {
   int nblk = 0;  // np / SB
   ChainState* chain = nullptr;  // device words of the one-launch recurrences (null: a launch per block)
   bool one_launch() const { return chain && SB <= kChainMaxSB; }
+  // parts per million of the current device's work-group slots (its
+  // occupancy for the recurrences' registers and LDS) that one launch of the
+  // one-launch recurrences needs; above 1e6 (or on error) they cannot run
+  long long chain_share() const;
   void plan(int n, int bw);
   size_t band_len() const;
   size_t tinv_len() const { return (size_t)np * 64; }

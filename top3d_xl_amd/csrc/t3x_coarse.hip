@@ -703,6 +703,23 @@ __global__ __launch_bounds__(CT) void k_solve_chain(int n, int SB, int nblk, con
 }
 
 // ---- host side -----------------------------------------------------------
+long long CoarseSolver::chain_share() const {
+  constexpr long long kNever = 2000000;
+  if (SB > kChainMaxSB) return kNever;
+  const int G = (SB + 3) / 4;
+  const size_t lds = (size_t)SB * sizeof(double);
+  int dev = 0, cus = 0, perCU = 0, perCU2 = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return kNever;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return kNever;
+  auto fwd = SB <= 4 * CT ? k_solve_chain<double, 4, true> : k_solve_chain<double, kChainMaxSB / CT, true>;
+  auto bwd = SB <= 4 * CT ? k_solve_chain<float, 4, false> : k_solve_chain<float, kChainMaxSB / CT, false>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fwd, CT, lds) != hipSuccess) return kNever;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU2, bwd, CT, lds) != hipSuccess) return kNever;
+  const long long slots = (long long)std::min(perCU, perCU2) * cus;
+  if (slots <= 0) return kNever;
+  return (1000000LL * G + slots - 1) / slots;
+}
+
 void CoarseSolver::plan(int n_, int bw_) {
   n = n_;
   bw = bw_;

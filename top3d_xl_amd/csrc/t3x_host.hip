@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -154,6 +155,10 @@ struct GroupTimer {
   long long count = 0;
 };
 
+// parts per million of the device's work-group slots held by this process's
+// handles for the one-launch coarsest recurrences
+static std::atomic<long long> gChainShare{0};
+
 struct t3x_problem {
   // Caller-provided resources (the serving engine's pooled slot): device
   // allocations are carved in order out of [arena, arena + arenaSize) instead
@@ -162,10 +167,10 @@ struct t3x_problem {
   char* arena = nullptr;
   size_t arenaSize = 0, arenaUsed = 0;
   bool ownStream = true, ownHostBuf = true;
-  // coarsest solve: one launch per recurrence (k_solve_chain) unless the
-  // lender runs other problems beside this one (its groups must all be
-  // resident together, which concurrent problems' copies would not ensure)
+  // coarsest solve: one launch per recurrence (k_solve_chain) where allowed
+  // (its groups must all be resident together: see gChainShare)
   bool oneLaunchCoarse = true;
+  long long chainShare = 0;  // its part (per million) of the device slots held for one-launch solves
   OptState opt;
   GroupTimer timers[TG_COUNT];
   t3x_params P{};
@@ -307,6 +312,7 @@ struct t3x_problem {
     T.pending.clear();
   }
   ~t3x_problem() {
+    if (chainShare) gChainShare.fetch_sub(chainShare);
     for (auto& g : pcgGraphs) (void)hipGraphExecDestroy(g.second);
     for (auto& T : timers) {
       for (auto& pr : T.pending) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -774,7 +780,19 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
     T3X_HIP(hipMemsetAsync(P.cvb, 0, 3 * (size_t)P.cs.np * sizeof(double), P.stream));
     T3X_HIP(hipMemsetAsync(P.chainSt, 0, sizeof(ChainState), P.stream));
     P.sync();
-    if (P.oneLaunchCoarse) P.cs.chain = P.chainSt;
+    // the handles of the process that use the one-launch form together
+    // need at most all of the device's work-group slots, so recurrences
+    // running at the same time can always all be resident
+    if (P.oneLaunchCoarse) {
+      const long long need = P.cs.chain_share();
+      long long held = gChainShare.load();
+      while (held + need <= 1000000 && !gChainShare.compare_exchange_weak(held, held + need)) {
+      }
+      if (held + need <= 1000000) {
+        P.cs.chain = P.chainSt;
+        P.chainShare = need;
+      }
+    }
   }
   // ---- compact maps (eleMapBack / nodMapBack order)
   for (long long n = 0; n < NN; ++n) {

@@ -18,9 +18,9 @@ struct PoolSlot {
   size_t size = 0;
   hipStream_t stream = nullptr;
   double* hostBuf = nullptr;  // 256 doubles, pinned
-  // coarsest solve in one launch per recurrence: only where no other problem
-  // can hold the device's work-group slots meanwhile (one active request)
-  bool oneLaunchCoarse = false;
+  // coarsest solve in one launch per recurrence where the process's share of
+  // the device's work-group slots allows it
+  bool oneLaunchCoarse = true;
 };
 
 // t3x_create with the built-in or explicit one-case boundary conditions, on

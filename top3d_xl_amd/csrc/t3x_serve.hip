@@ -126,7 +126,7 @@ void t3x_server::run_one(const Pending& p, const WorkerRes& w, Finished& f) {
   slot.size = rounded(p.bytes);
   slot.stream = w.stream;
   slot.hostBuf = w.hostBuf;
-  slot.oneLaunchCoarse = cfg.maxActive == 1;
+  slot.oneLaunchCoarse = true;  // bounded process-wide by device occupancy (t3x_host)
   int rc = t3x::create_pooled(vox.data(), ny, nx, nz, (int)fixN.size(), fixN.data(), fixS.data(), (int)loadN.size(),
                               loadN.data(), loadV.data(), &cfg.params, slot, &h);
   f.c.tag = r.tag;
