@@ -136,7 +136,10 @@ int t3x_set_passive_layers(t3x_problem* h, int numLayerBoundary, int numLayerLoa
  * the unpadded grid; at most cap are written to voxelIds (which may be NULL
  * to query the count). */
 int t3x_get_passive(const t3x_problem* h, int cap, int* voxelIds, int* count);
-/* info[0]=numLevels; per level l at 1+8l: resX,resY,resZ,span,numElements,numNodes,0,0 */
+/* info[0]=numLevels; per level l at 1+8l: resX,resY,resZ,span,numElements,numNodes,a,0;
+   a = 1 on the coarsest level when its solve runs as one launch per block
+   recurrence (granted while the process's handles fit the device's
+   work-group slots), else 0 */
 int t3x_info(const t3x_problem* h, int* info);
 
 /* Results of one optimisation (TOP3D_XL_TO 40-223 / TOP3D_XL_PIO 225-440).

@@ -241,6 +241,36 @@ def test_auto_depth_hierarchy_192():
     assert ec <= C_TOL and e1 <= C_TOL
 
 
+def test_one_launch_coarse_solve_bitwise_192():
+    """The coarsest solve as one launch per block recurrence (k_solve_chain)
+    against a launch per block, at the reference's 192x384x192 hierarchy in
+    FP64 (coarsest 91,875 DOF, solve blocks of 1,984 rows: the recurrences'
+    grid takes nearly all of the device's work-group slots, so the first
+    handle gets the one-launch form and the second one cannot): the same
+    V-cycle output, bit for bit."""
+    import top3d_xl_amd as t3x
+    vox = np.ones((192, 384, 192), bool)
+    print("\n[gpu-mixed] one-launch vs per-block coarsest solve, 192x384x192 FP64", flush=True)
+    a = t3x.Problem(vox)
+    b = t3x.Problem(vox)
+    try:
+        if not a.one_launch_coarse:
+            pytest.skip("another handle of this process holds the device's one-launch share")
+        assert not b.one_launch_coarse
+        E = 1e-6 + np.random.default_rng(3).random(a.nE) ** 3
+        r = np.random.default_rng(4).standard_normal(3 * a.nN)
+        outs = []
+        for m in (a, b):
+            m.set_modulus(E)
+            m.assemble()
+            outs.append(m.vcycle(r))
+        print(f"  |z| {np.abs(outs[0]).max():.3e}")
+        assert np.array_equal(outs[0], outs[1])
+    finally:
+        a.close()
+        b.close()
+
+
 def test_standard_vcycle_matches_numpy():
     """The FP32 'Standard' cycle against a float64 numpy V(1,1) on the brute-force
     Galerkin hierarchy (tests/bruteforce.py; independent of the oracle)."""

@@ -1604,6 +1604,7 @@ int t3x_info(const t3x_problem* h, int* info) {
     int* o = info + 1 + 8 * l;
     o[0] = V.nx; o[1] = V.ny; o[2] = V.nz; o[3] = V.span; o[4] = V.nCells; o[5] = V.nNodes; o[6] = 0; o[7] = 0;
   }
+  if (h->L > 0) info[1 + 8 * (h->L - 1) + 6] = h->cs.one_launch() ? 1 : 0;
   return T3X_OK;
 }
 

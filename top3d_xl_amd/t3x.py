@@ -196,6 +196,8 @@ class Problem:
                                     numElements=int(o[4]), numNodes=int(o[5])))
         self.nE = self.levels[0]["numElements"]
         self.nN = self.levels[0]["numNodes"]
+        # the coarsest solve runs as one launch per block recurrence
+        self.one_launch_coarse = bool(info[1 + 8 * (self.num_levels - 1) + 6])
 
     def close(self):
         if getattr(self, "h", None):
