@@ -281,8 +281,8 @@ def main():
             VARIANTS[k] = dict(VARIANTS[k], coarsestControl=args.coarsest_control)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build measures one GPU only: the boxes here have one MI355X and the "
-                                   "multi-GPU design (x-slab decomposition) is documented, not run",
+        print(json.dumps({"error": "one GPU only: the solver runs on one MI355X; the multi-GPU design "
+                                   "(x-slab decomposition) is documented, not run",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.size is None:
