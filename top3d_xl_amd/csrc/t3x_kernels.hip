@@ -38,11 +38,12 @@ inline int grid_for(long long n) {
 // One-block finalisers: fold a reduction's partials (fixed order) and apply
 // the scalar logic of the solver step that consumes it.
 // =========================================================================
-__global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* __restrict__ partials, double* sc,
-                                                 int* fl, double* out) {
+constexpr int kFinBlock = 1024;  // the finaliser's threads (wide folds of up to kMaxPartials partials)
+__global__ __launch_bounds__(kFinBlock) void k_fin1(int op, int nb, const double* __restrict__ partials, double* sc,
+                                                    int* fl, double* out) {
   if (fl && fl[FL_DONE]) return;
   double v[1];
-  fold_partials<1>(partials, nb, v);
+  fold_partials<1, kFinBlock>(partials, nb, v);
   if (threadIdx.x != 0) return;
   const double t = v[0];
   switch (op) {
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_fin1(int op, int nb, const double* _
   }
 }
 void launch_fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out) {
-  hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kBlock), 0, s, op, nb, ws.partials, sc, fl, out);
+  hipLaunchKernelGGL(k_fin1, dim3(1), dim3(kFinBlock), 0, s, op, nb, ws.partials, sc, fl, out);
 }
 static void fin1(hipStream_t s, int op, int nb, const RedWs& ws, double* sc, int* fl, double* out) {
   launch_fin1(s, op, nb, ws, sc, fl, out);
@@ -675,20 +676,20 @@ void launch_set_vals(hipStream_t s, double* dst, int n, const double* vals) {
 __global__ void k_set_int(int* dst, int v) { *dst = v; }
 void launch_set_int(hipStream_t s, int* dst, int v) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dst, v); }
 void launch_fill(hipStream_t s, double* p, double v, long long n) {
-  hipLaunchKernelGGL(k_fill, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, p, v, n);
+  hipLaunchKernelGGL(k_fill, dim3(stream_blocks(n)), dim3(kBlock), 0, s, p, v, n);
 }
 __global__ void k_copy(double* d, const double* a, long long n) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) d[i] = a[i];
 }
 void launch_copy(hipStream_t s, double* dst, const double* src, long long n) {
-  hipLaunchKernelGGL(k_copy, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
+  hipLaunchKernelGGL(k_copy, dim3(stream_blocks(n)), dim3(kBlock), 0, s, dst, src, n);
 }
 __global__ void k_sub(double* r, const double* b, const double* q, long long n) {
   for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
     r[i] = b[i] - q[i];
 }
 void launch_sub(hipStream_t s, double* r, const double* b, const double* q, long long n) {
-  hipLaunchKernelGGL(k_sub, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, b, q, n);
+  hipLaunchKernelGGL(k_sub, dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, b, q, n);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a, const double* __restrict__ b,
@@ -739,9 +740,9 @@ void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, con
                        const RedWs& ws, double* sc, int* fl, const float* dj, float* xj, double omega, Act act) {
   (void)y;
   (void)p;
-  hipLaunchKernelGGL(k_pcg_update, dim3(red_blocks(n)), dim3(kBlock), 0, s, r, q, n, ws, sc, fl, dj, xj, omega,
+  hipLaunchKernelGGL(k_pcg_update, dim3(red1_blocks(n)), dim3(kBlock), 0, s, r, q, n, ws, sc, fl, dj, xj, omega,
                      act);
-  fin1(s, FIN_RR, red_blocks(n), ws, sc, fl, nullptr);
+  fin1(s, FIN_RR, red1_blocks(n), ws, sc, fl, nullptr);
 }
 
 // y += lambda p (616) for the iteration just finished, then p = z + beta p (630).
@@ -758,7 +759,7 @@ __global__ void k_pcg_pupdate(double* __restrict__ y, double* __restrict__ p, co
 }
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
                         const int* fl, Act act) {
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, z, n, sc, fl, act);
+  hipLaunchKernelGGL(k_pcg_pupdate, dim3(stream_blocks(n)), dim3(kBlock), 0, s, y, p, z, n, sc, fl, act);
 }
 // After the loop: the converging iteration skipped its p-update, so its
 // y += lambda p is applied here (no-op if the solve hit maxIT).
@@ -771,7 +772,7 @@ __global__ void k_pcg_finish(double* __restrict__ y, const double* __restrict__ 
 }
 void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl,
                        Act act) {
-  hipLaunchKernelGGL(k_pcg_finish, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, n, sc, fl, act);
+  hipLaunchKernelGGL(k_pcg_finish, dim3(stream_blocks(n)), dim3(kBlock), 0, s, y, p, n, sc, fl, act);
 }
 
 // Mixed precision: the V-cycle output z is FP32; y, r, p and scalars FP64.
@@ -788,7 +789,7 @@ __global__ void k_pcg_pupdate_mixed(double* __restrict__ y, double* __restrict__
 }
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const float* z, long long n, const double* sc,
                         const int* fl, Act act) {
-  hipLaunchKernelGGL(k_pcg_pupdate_mixed, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, y, p, z, n, sc, fl, act);
+  hipLaunchKernelGGL(k_pcg_pupdate_mixed, dim3(stream_blocks(n)), dim3(kBlock), 0, s, y, p, z, n, sc, fl, act);
 }
 template <typename TD, typename TS>
 __global__ void k_convert(TD* __restrict__ dst, const TS* __restrict__ src, long long n) {
@@ -796,10 +797,10 @@ __global__ void k_convert(TD* __restrict__ dst, const TS* __restrict__ src, long
     dst[i] = (TD)src[i];
 }
 void launch_convert(hipStream_t s, float* dst, const double* src, long long n) {
-  hipLaunchKernelGGL((k_convert<float, double>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, dst, src, n);
 }
 void launch_convert(hipStream_t s, double* dst, const float* src, long long n) {
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, dst, src, n);
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, dst, src, n);
 }
 
 __global__ void k_pcg_init(double* sc, int* fl, double tol) {
@@ -1508,7 +1509,7 @@ __global__ void k_set_where(long long n, const uint8_t* __restrict__ mask, doubl
     if (mask[i]) a[i] = v;
 }
 void launch_set_where(hipStream_t s, long long n, const uint8_t* mask, double* a, double v) {
-  hipLaunchKernelGGL(k_set_where, dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, n, mask, a, v);
+  hipLaunchKernelGGL(k_set_where, dim3(stream_blocks(n)), dim3(kBlock), 0, s, n, mask, a, v);
 }
 
 }  // namespace t3x

@@ -302,17 +302,17 @@ void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, c
 
 void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
                     const int* fl, Act act) {
-  hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
+  hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, d, x, omega,
                      n, fl, act);
 }
 void launch_jacobi0(hipStream_t s, const double* r, const double* d, double* x, double omega, long long n,
                     const int* fl, Act act) {
-  hipLaunchKernelGGL((k_jacobi0<double, double, double>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x,
+  hipLaunchKernelGGL((k_jacobi0<double, double, double>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, d, x,
                      omega, n, fl, act);
 }
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
                     const int* fl, Act act) {
-  hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(red_blocks(n) * 4), dim3(kBlock), 0, s, r, d, x, omega,
+  hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, d, x, omega,
                      n, fl, act);
 }
 
