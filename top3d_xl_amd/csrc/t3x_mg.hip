@@ -20,6 +20,8 @@
 // output) from the same code.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "t3x_kernels.h"
 #include "t3x_reduce.h"
 
@@ -66,6 +68,53 @@ __global__ __launch_bounds__(kBlock) void k_restrict_y(Geo gf, int NYc, const TI
   tmp[2 * TN + o] = (TM)t2;
 }
 
+// The y pass through LDS: block b takes the rows [R b, R b + R) (row
+// x + NX z: NY fine values and NYc intermediate ones, both contiguous), loads
+// the fine rows into LDS with coalesced loads (values of void 16-node chunks
+// are 0 and not read), then computes one output per thread from LDS with the
+// weights and summation order of k_restrict_y, so the result is the same.
+// The direct kernel's loads are stride S across lanes, each fine value read
+// by 2 S - 1 threads.
+template <int S, typename TI, typename TM>
+__global__ __launch_bounds__(kBlock) void k_restrict_y_lds(Geo gf, int NYc, int R, const TI* __restrict__ rf,
+                                                           TM* __restrict__ tmp, const int* fl, Act actF) {
+  if (fl && fl[FL_DONE]) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  TI* sv = (TI*)smem;  // [component][R NY]
+  const long long rows = (long long)gf.NX * gf.NZ;
+  const long long r0 = (long long)blockIdx.x * R;
+  const int nr = (int)min((long long)R, rows - r0);
+  const int len = nr * gf.NY, cs = R * gf.NY;
+  const long long f0 = r0 * gf.NY;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    for (int i = threadIdx.x; i < len; i += kBlock) {
+      const long long f = f0 + i;
+      sv[c * cs + i] = (actF.chunk && !actF.chunk[f >> 4]) ? TI(0) : rf[c * gf.NN + f];
+    }
+  __syncthreads();
+  const long long TN = (long long)NYc * rows;
+  const int nout = nr * NYc;
+  for (int o = threadIdx.x; o < nout; o += kBlock) {
+    const int rr = o / NYc, Jy = o - rr * NYc;
+    const TI* row = sv + rr * gf.NY;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+    for (int dy = -(S - 1); dy <= S - 1; ++dy) {
+      const int y = S * Jy + dy;
+      if (y < 0 || y >= gf.NY) continue;
+      const double w = hat_w<S>(dy);
+      t0 += w * (double)row[y];
+      t1 += w * (double)row[cs + y];
+      t2 += w * (double)row[2 * cs + y];
+    }
+    const long long oo = r0 * NYc + o;
+    tmp[oo] = (TM)t0;
+    tmp[TN + oo] = (TM)t1;
+    tmp[2 * TN + oo] = (TM)t2;
+  }
+}
+
 // x-z pass: rc[c](J) = mask(J) * sum_dz w(dz) sum_dx w(dx) tmp[c](Jy, S*Jx+dx, S*Jz+dz)
 template <int S, typename TM, typename TO>
 __global__ __launch_bounds__(kBlock) void k_restrict_xz(Geo gf, Geo gc, const TM* __restrict__ tmp,
@@ -110,16 +159,21 @@ __global__ __launch_bounds__(kBlock) void k_restrict_xz(Geo gf, Geo gc, const TM
 
 // out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere (758,
 // 770, 773); optional fused z.r partials (629).  Block (bx, bz) covers 256
-// (y, x) columns and the node planes [bz*zc, bz*zc + zc); every load is in
-// bounds (zero weights read a clamped neighbour), so the loads of a plane
-// issue together and the mask only selects the result.
+// (y, x) columns and the node planes [bz*zc, bz*zc + zc), zc a multiple of
+// kPZ.  A thread takes its planes kPZ at a time: the masks first, then every
+// load the solid ones need (the coarse values of the group's coarse z
+// layers, the fine operands) issued together, then the arithmetic — two
+// dependent memory round trips per kPZ planes.
 // ADD: out += P xc instead (the 'Standard' cycle's coarse correction, 770); rf
 // and diag are then unused.
+constexpr int kPZ = 4;
 template <int S, typename TX, typename TR, typename TD, typename TO, bool ADD = false>
 __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega, int zc, const TX* __restrict__ xc,
                                                     const TR* __restrict__ rf, const TD* __restrict__ diag,
                                                     const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
                                                     double* __restrict__ partials, const int* fl) {
+  static_assert(kPZ % S == 0, "a plane group spans whole coarse intervals");
+  constexpr int NL = kPZ / S + 1;  // coarse z layers of a plane group
   if (fl && fl[FL_DONE]) return;
   const int plane = gf.NY * gf.NX;
   const long long NN = gf.NN, NC = gc.NN;
@@ -136,47 +190,70 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
     const long long cyx[4] = {Jy + (long long)gc.NY * Jx, Jy1 + (long long)gc.NY * Jx, Jy + (long long)gc.NY * Jx1,
                               Jy1 + (long long)gc.NY * Jx1};
     const int z0 = blockIdx.y * zc, z1 = min(gf.NZ, z0 + zc);
-    // the 8 coarse values of this column's cell, reloaded only when a solid
-    // node needs a coarse z interval other than the loaded one
-    double cv[8][3];
-    int cJz = -1;
-    for (int z = z0; z < z1; ++z) {
-      const long long n = idx + (long long)plane * z;
-      const bool solid = nmaskF[n] != 0;
-      const int Jz = z / S, tz = z - S * Jz;
-      if (solid && Jz != cJz) {
-        cJz = Jz;
-        const int Jz1 = min(Jz + 1, gc.NZ - 1);
-        const long long zo[2] = {planeC * Jz, planeC * Jz1};
+    for (int zb = z0; zb < z1; zb += kPZ) {
+      bool solid[kPZ];
+      bool any = false;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const long long J = cyx[(c & 1) + 2 * ((c >> 1) & 1)] + zo[c >> 2];
-          cv[c][0] = (double)xc[J];
-          cv[c][1] = (double)xc[NC + J];
-          cv[c][2] = (double)xc[2 * NC + J];
-        }
+      for (int i = 0; i < kPZ; ++i) {
+        solid[i] = zb + i < z1 && nmaskF[idx + (long long)plane * (zb + i)] != 0;
+        any |= solid[i];
       }
       // void nodes hold 0 in every level vector (all of their writers leave
       // them 0): no load, no store, no interpolation
-      if (solid) {
+      if (!any) continue;
+      TX cv[NL][4][3];  // coarse layers Jz0 .. Jz0 + NL - 1 (clamped; a clamped one has weight 0)
+      const int Jz0 = zb / S;
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        const long long zo = planeC * min(Jz0 + l, gc.NZ - 1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const long long J = cyx[c] + zo;
+          cv[l][c][0] = xc[J];
+          cv[l][c][1] = xc[NC + J];
+          cv[l][c][2] = xc[2 * NC + J];
+        }
+      }
+      TO ov[kPZ][3];
+      TR rv[kPZ][3];
+      TD dv[kPZ][3];
+#pragma unroll
+      for (int i = 0; i < kPZ; ++i) {
+        if (!solid[i]) continue;
+        const long long n = idx + (long long)plane * (zb + i);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (ADD) {
+            ov[i][k] = out[k * NN + n];
+          } else {
+            rv[i][k] = rf[k * NN + n];
+            dv[i][k] = diag[k * NN + n];
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kPZ; ++i) {
+        if (!solid[i]) continue;
+        const long long n = idx + (long long)plane * (zb + i);
+        const int l0 = i / S, tz = i - S * l0;  // zb is a multiple of S: plane zb + i lies in layer Jz0 + i / S
         const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
         double p[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
           const double w = wy[a] * wx[b] * wz[d];
-          p[0] += w * cv[c][0];
-          p[1] += w * cv[c][1];
-          p[2] += w * cv[c][2];
+          const int cc = a + 2 * b;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) p[k] += w * (double)(d ? cv[l0 + 1][cc][k] : cv[l0][cc][k]);
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           if (ADD) {
-            out[k * NN + n] = solid ? (TO)((double)out[k * NN + n] + p[k]) : TO(0);
+            out[k * NN + n] = (TO)((double)ov[i][k] + p[k]);
           } else {
-            const double r = (double)rf[k * NN + n];
-            const double dd = solid ? (double)diag[k * NN + n] : 1.0;
-            const TO v = solid ? (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd) : TO(0);
+            const double r = (double)rv[i][k];
+            const double dd = (double)dv[i][k];
+            const TO v = (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd);
             zr += (double)v * r;
             out[k * NN + n] = v;
           }
@@ -191,17 +268,32 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
   }
 }
 
+constexpr size_t kRestrictLds = 48 * 1024;  // LDS bytes of the y pass's rows at most
+
 template <typename TI, typename TM, typename TO>
 void restrict_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const TI* rf, TM* tmp,
                    const uint8_t* nmaskC, TO* rc, const int* fl, Act actF) {
   const int plane = gc.NY * gf.NX;
   const dim3 g1((plane + kBlock - 1) / kBlock, gf.NZ);
   const dim3 g2((gc.NY * gc.NX + kBlock - 1) / kBlock, gc.NZ);
+  // the LDS y pass: about one intermediate value per thread (R rows a block)
+  // while the rows fit kRestrictLds bytes
+  const int R = std::max(1, (kBlock + gc.NY / 2) / gc.NY);
+  const size_t lds = (size_t)R * gf.NY * 3 * sizeof(TI);
+  const long long rows = (long long)gf.NX * gf.NZ;
+  const bool viaLds = lds <= kRestrictLds && (rows + R - 1) / R <= 0x7fffffffLL;
+  const dim3 gl((unsigned)((rows + R - 1) / R));
   if (span == 4) {
-    hipLaunchKernelGGL((k_restrict_y<4, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl, actF);
+    if (viaLds)
+      hipLaunchKernelGGL((k_restrict_y_lds<4, TI, TM>), gl, dim3(kBlock), lds, s, gf, gc.NY, R, rf, tmp, fl, actF);
+    else
+      hipLaunchKernelGGL((k_restrict_y<4, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl, actF);
     hipLaunchKernelGGL((k_restrict_xz<4, TM, TO>), g2, dim3(kBlock), 0, s, gf, gc, tmp, nmaskC, rc, fl);
   } else {
-    hipLaunchKernelGGL((k_restrict_y<2, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl, actF);
+    if (viaLds)
+      hipLaunchKernelGGL((k_restrict_y_lds<2, TI, TM>), gl, dim3(kBlock), lds, s, gf, gc.NY, R, rf, tmp, fl, actF);
+    else
+      hipLaunchKernelGGL((k_restrict_y<2, TI, TM>), g1, dim3(kBlock), 0, s, gf, gc.NY, rf, tmp, fl, actF);
     hipLaunchKernelGGL((k_restrict_xz<2, TM, TO>), g2, dim3(kBlock), 0, s, gf, gc, tmp, nmaskC, rc, fl);
   }
 }
