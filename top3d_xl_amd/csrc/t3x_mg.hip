@@ -86,12 +86,26 @@ __global__ __launch_bounds__(kBlock) void k_restrict_y_lds(Geo gf, int NYc, int 
   const int nr = (int)min((long long)R, rows - r0);
   const int len = nr * gf.NY, cs = R * gf.NY;
   const long long f0 = r0 * gf.NY;
+  constexpr int UL = 4;  // fine values per component and thread in flight
+  for (int i0 = threadIdx.x; i0 < len; i0 += UL * kBlock) {
+    TI v[3][UL];
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-    for (int i = threadIdx.x; i < len; i += kBlock) {
+    for (int u = 0; u < UL; ++u) {
+      const int i = i0 + u * kBlock;
       const long long f = f0 + i;
-      sv[c * cs + i] = (actF.chunk && !actF.chunk[f >> 4]) ? TI(0) : rf[c * gf.NN + f];
+      const bool on = i < len && !(actF.chunk && !actF.chunk[f >> 4]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c][u] = on ? rf[c * gf.NN + f] : TI(0);
     }
+#pragma unroll
+    for (int u = 0; u < UL; ++u) {
+      const int i = i0 + u * kBlock;
+      if (i < len) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sv[c * cs + i] = v[c][u];
+      }
+    }
+  }
   __syncthreads();
   const long long TN = (long long)NYc * rows;
   const int nout = nr * NYc;
@@ -159,19 +173,94 @@ __global__ __launch_bounds__(kBlock) void k_restrict_xz(Geo gf, Geo gc, const TM
 
 // out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere (758,
 // 770, 773); optional fused z.r partials (629).  Block (bx, bz) covers 256
-// (y, x) columns and the node planes [bz*zc, bz*zc + zc), zc a multiple of
-// kPZ.  A thread takes its planes kPZ at a time: the masks first, then every
-// load the solid ones need (the coarse values of the group's coarse z
-// layers, the fine operands) issued together, then the arithmetic — two
-// dependent memory round trips per kPZ planes.
-// ADD: out += P xc instead (the 'Standard' cycle's coarse correction, 770); rf
-// and diag are then unused.
-constexpr int kPZ = 4;
-template <int S, typename TX, typename TR, typename TD, typename TO, bool ADD = false>
-__global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega, int zc, const TX* __restrict__ xc,
+// (y, x) columns and the node planes [bz*zc, bz*zc + zc); every load is in
+// bounds (zero weights read a clamped neighbour), so the loads of a plane
+// issue together and the mask only selects the result.
+template <int S, typename TX, typename TR, typename TD, typename TO>
+__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(Geo gf, Geo gc, double omega, int zc, const TX* __restrict__ xc,
                                                     const TR* __restrict__ rf, const TD* __restrict__ diag,
                                                     const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
                                                     double* __restrict__ partials, const int* fl) {
+  if (fl && fl[FL_DONE]) return;
+  const int plane = gf.NY * gf.NX;
+  const long long NN = gf.NN, NC = gc.NN;
+  const long long planeC = (long long)gc.NY * gc.NX;
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  double zr = 0.0;
+  if (idx < plane) {
+    const int x = idx / gf.NY, y = idx - x * gf.NY;
+    const int Jy = y / S, Jx = x / S;
+    const int ty = y - S * Jy, tx = x - S * Jx;
+    const double wy[2] = {(double)(S - ty) * (1.0 / S), (double)ty * (1.0 / S)};
+    const double wx[2] = {(double)(S - tx) * (1.0 / S), (double)tx * (1.0 / S)};
+    const int Jy1 = min(Jy + 1, gc.NY - 1), Jx1 = min(Jx + 1, gc.NX - 1);
+    const long long cyx[4] = {Jy + (long long)gc.NY * Jx, Jy1 + (long long)gc.NY * Jx, Jy + (long long)gc.NY * Jx1,
+                              Jy1 + (long long)gc.NY * Jx1};
+    const int z0 = blockIdx.y * zc, z1 = min(gf.NZ, z0 + zc);
+    // the 8 coarse values of this column's cell, reloaded only when a solid
+    // node needs a coarse z interval other than the loaded one
+    double cv[8][3];
+    int cJz = -1;
+    for (int z = z0; z < z1; ++z) {
+      const long long n = idx + (long long)plane * z;
+      const bool solid = nmaskF[n] != 0;
+      const int Jz = z / S, tz = z - S * Jz;
+      if (solid && Jz != cJz) {
+        cJz = Jz;
+        const int Jz1 = min(Jz + 1, gc.NZ - 1);
+        const long long zo[2] = {planeC * Jz, planeC * Jz1};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const long long J = cyx[(c & 1) + 2 * ((c >> 1) & 1)] + zo[c >> 2];
+          cv[c][0] = (double)xc[J];
+          cv[c][1] = (double)xc[NC + J];
+          cv[c][2] = (double)xc[2 * NC + J];
+        }
+      }
+      // void nodes hold 0 in every level vector (all of their writers leave
+      // them 0): no load, no store, no interpolation
+      if (solid) {
+        const double wz[2] = {(double)(S - tz) * (1.0 / S), (double)tz * (1.0 / S)};
+        double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int a = c & 1, b = (c >> 1) & 1, d = c >> 2;
+          const double w = wy[a] * wx[b] * wz[d];
+          p[0] += w * cv[c][0];
+          p[1] += w * cv[c][1];
+          p[2] += w * cv[c][2];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double r = (double)rf[k * NN + n];
+          const double dd = solid ? (double)diag[k * NN + n] : 1.0;
+          const TO v = solid ? (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd) : TO(0);
+          zr += (double)v * r;
+          out[k * NN + n] = v;
+        }
+      }
+    }
+  }
+  if (partials) {
+    double v[1] = {zr};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) partials[blockIdx.x + (size_t)gridDim.x * blockIdx.y] = v[0];
+  }
+}
+
+// out += P xc on solid nodes (the 'Standard' cycle's coarse correction, 770),
+// the grid of k_prolong_jacobi.  A thread takes its planes kPZ at a time: the
+// masks first, then every load the solid ones need (the coarse values of the
+// group's coarse z layers, the fine values) issued together, then the
+// arithmetic — two dependent memory round trips per kPZ planes (fine
+// 256x512x256 FP32: 199 against 290 us plane by plane; the Jacobi form above,
+// with twice the fine operands, keeps its plane-by-plane loop: grouped, its
+// FP64 instance needs 212 VGPRs and ran 531 against 461 us).
+constexpr int kPZ = 4;
+template <int S, typename TX, typename TO>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(Geo gf, Geo gc, int zc, const TX* __restrict__ xc,
+                                                        const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
+                                                        const int* fl) {
   static_assert(kPZ % S == 0, "a plane group spans whole coarse intervals");
   constexpr int NL = kPZ / S + 1;  // coarse z layers of a plane group
   if (fl && fl[FL_DONE]) return;
@@ -179,7 +268,6 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
   const long long NN = gf.NN, NC = gc.NN;
   const long long planeC = (long long)gc.NY * gc.NX;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
-  double zr = 0.0;
   if (idx < plane) {
     const int x = idx / gf.NY, y = idx - x * gf.NY;
     const int Jy = y / S, Jx = x / S;
@@ -215,21 +303,12 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
         }
       }
       TO ov[kPZ][3];
-      TR rv[kPZ][3];
-      TD dv[kPZ][3];
 #pragma unroll
       for (int i = 0; i < kPZ; ++i) {
         if (!solid[i]) continue;
         const long long n = idx + (long long)plane * (zb + i);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (ADD) {
-            ov[i][k] = out[k * NN + n];
-          } else {
-            rv[i][k] = rf[k * NN + n];
-            dv[i][k] = diag[k * NN + n];
-          }
-        }
+        for (int k = 0; k < 3; ++k) ov[i][k] = out[k * NN + n];
       }
 #pragma unroll
       for (int i = 0; i < kPZ; ++i) {
@@ -247,24 +326,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong(Geo gf, Geo gc, double omega
           for (int k = 0; k < 3; ++k) p[k] += w * (double)(d ? cv[l0 + 1][cc][k] : cv[l0][cc][k]);
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          if (ADD) {
-            out[k * NN + n] = (TO)((double)ov[i][k] + p[k]);
-          } else {
-            const double r = (double)rv[i][k];
-            const double dd = (double)dv[i][k];
-            const TO v = (TO)((omega * (r / dd) + p[k]) + (omega * r) / dd);
-            zr += (double)v * r;
-            out[k * NN + n] = v;
-          }
-        }
+        for (int k = 0; k < 3; ++k) out[k * NN + n] = (TO)((double)ov[i][k] + p[k]);
       }
     }
-  }
-  if (partials) {
-    double v[1] = {zr};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) partials[blockIdx.x + (size_t)gridDim.x * blockIdx.y] = v[0];
   }
 }
 
@@ -298,11 +362,12 @@ void restrict_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, const 
   }
 }
 
-// grid (y-x tiles, z chunks) and the planes per chunk; at most kMaxPartials
-// blocks (a node plane beyond that is refused at set-up: prolong_plane_fits)
+// grid (y-x tiles, z chunks) and the planes per chunk (a multiple of kPZ);
+// at most kMaxPartials blocks (a node plane beyond that is refused at set-up:
+// prolong_plane_fits)
 dim3 prolong_grid(const Geo& gf, int* zc) {
   const int bx = (gf.NY * gf.NX + kBlock - 1) / kBlock;
-  int c = 4;
+  int c = kPZ;
   while ((long long)bx * ((gf.NZ + c - 1) / c) > kMaxPartials && c < gf.NZ) c *= 2;
   *zc = c;
   return dim3(bx, (gf.NZ + c - 1) / c);
@@ -317,10 +382,10 @@ void prolong_impl(hipStream_t s, const Geo& gf, const Geo& gc, int span, double 
   const dim3 g = prolong_grid(gf, &zc);
   double* part = mode ? ws->partials : nullptr;
   if (span == 4)
-    hipLaunchKernelGGL((k_prolong<4, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, zc, xc, rf, diag,
+    hipLaunchKernelGGL((k_prolong_jacobi<4, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, zc, xc, rf, diag,
                        nmaskF, out, part, fl);
   else
-    hipLaunchKernelGGL((k_prolong<2, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, zc, xc, rf, diag,
+    hipLaunchKernelGGL((k_prolong_jacobi<2, TX, TR, TD, TO>), g, dim3(kBlock), 0, s, gf, gc, omega, zc, xc, rf, diag,
                        nmaskF, out, part, fl);
   if (mode) launch_fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, (int)(g.x * g.y), *ws, sc, (int*)fl, nullptr);
 }
@@ -377,11 +442,9 @@ static void prolong_add_impl(hipStream_t s, const Geo& gf, const Geo& gc, int sp
   int zc = 0;
   const dim3 g = prolong_grid(gf, &zc);
   if (span == 4)
-    hipLaunchKernelGGL((k_prolong<4, T, T, T, T, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
-                       (const T*)nullptr, (const T*)nullptr, nmaskF, x, (double*)nullptr, fl);
+    hipLaunchKernelGGL((k_prolong_add<4, T, T>), g, dim3(kBlock), 0, s, gf, gc, zc, xc, nmaskF, x, fl);
   else
-    hipLaunchKernelGGL((k_prolong<2, T, T, T, T, true>), g, dim3(kBlock), 0, s, gf, gc, 0.0, zc, xc,
-                       (const T*)nullptr, (const T*)nullptr, nmaskF, x, (double*)nullptr, fl);
+    hipLaunchKernelGGL((k_prolong_add<2, T, T>), g, dim3(kBlock), 0, s, gf, gc, zc, xc, nmaskF, x, fl);
 }
 void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, const float* xc, float* x,
                         const uint8_t* nmaskF, const int* fl) {
