@@ -1083,11 +1083,11 @@ constexpr int kOcCand = (1 << kOcDepth) - 1;
 
 // x .* sqrt(-dc ./ lmid) (149) as t * rl with t = x sqrt(-dc) formed once per
 // cell and rl = 1/sqrt(lmid) once per candidate, then the move limits (150).
+constexpr double kOcMove = 0.2;
 __device__ __forceinline__ double oc_clamp(double x, double v) {
-  const double move = 0.2;
-  v = fmin(x + move, v);
+  v = fmin(x + kOcMove, v);
   v = fmin(1.0, v);
-  v = fmax(x - move, v);
+  v = fmax(x - kOcMove, v);
   return fmax(0.0, v);
 }
 
@@ -1121,19 +1121,24 @@ __global__ __launch_bounds__(kBlock) void k_oc_pass(long long n, const uint8_t* 
     const double xi = x[i];
     const double t = xi * sqrt(-dc[i]);
     const bool pas = passive && passive[i];
+    // oc_clamp's four nested min/max as one two-sided clamp to per-cell
+    // bounds: min and max are exact, so max(0, max(x - m, min(1, min(x + m,
+    // v)))) = max(lo, min(hi, v)) bit for bit (a NaN v yields max(lo, hi)
+    // either way); a passive cell's bounds are both 1 (xnew = 1)
+    const double hi = pas ? 1.0 : fmin(1.0, xi + kOcMove), lo = pas ? 1.0 : fmax(0.0, xi - kOcMove);
 #pragma unroll
     for (int k = 0; k < kOcCand; ++k) {
-      const double xn = pas ? 1.0 : oc_clamp(xi, t * rl[k]);
+      const double xn = fmax(lo, fmin(hi, t * rl[k]));
       s[k] += 1.0 * (xn - xi);  // 151
     }
   }
   write_partials<kOcCand>(s, ws.partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_oc_fin(int nb, const double* __restrict__ partials, double* oc) {
+__global__ __launch_bounds__(kFinBlock) void k_oc_fin(int nb, const double* __restrict__ partials, double* oc) {
   if (oc[3] != 0.0) return;
   double s[kOcCand];
-  fold_partials<kOcCand>(partials, nb, s);
+  fold_partials<kOcCand, kFinBlock>(partials, nb, s);
   if (threadIdx.x != 0) return;
   const double l1 = oc[0], l2 = oc[1];
   double a = l1, b = l2;
@@ -1160,7 +1165,7 @@ __global__ __launch_bounds__(kBlock) void k_oc_fin(int nb, const double* __restr
 void launch_oc_pass(hipStream_t s, long long n, const uint8_t* emask, const uint8_t* passive, const double* x,
                     const double* dc, const RedWs& ws, double* oc, int) {
   hipLaunchKernelGGL(k_oc_pass, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, passive, x, dc, ws, oc);
-  hipLaunchKernelGGL(k_oc_fin, dim3(1), dim3(kBlock), 0, s, red_blocks(n), ws.partials, oc);
+  hipLaunchKernelGGL(k_oc_fin, dim3(1), dim3(kFinBlock), 0, s, red_blocks(n), ws.partials, oc);
 }
 
 // xnew at the last evaluated lmid (149-150), change = max|xnew-x| (154), x = xnew.
