@@ -1241,11 +1241,11 @@ void compliance_cases(t3x_problem& P) {
   const uint8_t* em = P.lv[0].emask;
   const Geo g0 = geo(P.lv[0]);
   if (P.nCases == 1) {
-    launch_compliance(s, g0, P.dKe, em, P.Uc[0], P.ce);
+    launch_compliance_sf(s, g0, P.sf, em, P.Uc[0], P.ce);
     return;
   }
   for (int j = 0; j < P.nCases; ++j) {
-    launch_compliance(s, g0, P.dKe, em, P.Uc[j], P.ceTmp);
+    launch_compliance_sf(s, g0, P.sf, em, P.Uc[j], P.ceTmp);
     launch_elementwise(s, j == 0 ? OP_MULC : OP_AXPYC, P.NN(), em, P.ce, P.ceTmp, nullptr, P.wObj[j], 0, 0);
   }
 }
@@ -1906,7 +1906,7 @@ int t3x_pcg(t3x_problem* h, const double* b, double* x, double tol, int maxIT, i
 int t3x_compliance(t3x_problem* h, const double* U, double* ce) {
   return guarded(h, [&] {
     upload_nodal(*h, U, h->p);
-    launch_compliance(h->stream, geo(h->lv[0]), h->dKe, h->lv[0].emask, h->p, h->ce);
+    launch_compliance_sf(h->stream, geo(h->lv[0]), h->sf, h->lv[0].emask, h->p, h->ce);
     download_cells(*h, h->ce, ce);
     return (int)T3X_OK;
   });

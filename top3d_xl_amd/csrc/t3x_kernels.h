@@ -86,8 +86,9 @@ void launch_fine_smooth32_sf(hipStream_t s, const Geo& g, const SfCoef& cf, cons
 // fine-level K*u (y = Ku, with the PCG's p.q reduction when pcgStep)
 void launch_apply_K(hipStream_t s, const Geo& g, const SfCoef& cf, const double* E, const uint8_t* fix,
                     const double* u, double* y, const RedWs& ws, double* sc, const int* fl, bool pcgStep);
-void launch_compliance(hipStream_t s, const Geo& g, const double* Ke, const uint8_t* emask, const double* u,
-                       double* ce);
+// unit compliance per cell, u_e^T Ke u_e (579-598), sum-factorised
+void launch_compliance_sf(hipStream_t s, const Geo& g, const SfCoef& cf, const uint8_t* emask, const double* u,
+                          double* ce);
 void launch_diag_fine(hipStream_t s, const Geo& g, const double* Ke, const double* E, const uint8_t* fix,
                       const uint8_t* nmask, double* diag);
 void launch_diag_coarse(hipStream_t s, const Geo& g, const double* Ks, const uint8_t* emask,

@@ -102,42 +102,6 @@ void launch_apply_K(hipStream_t s, const Geo& g, const SfCoef& cf, const double*
 }
 
 // =========================================================================
-// Unit compliance ce_e = u_e^T Ke u_e with the unmodified Ke (579-598).
-// =========================================================================
-__global__ __launch_bounds__(kBlock) void k_compliance(Geo g, const double* __restrict__ Ke,
-                                                       const uint8_t* __restrict__ emask,
-                                                       const double* __restrict__ u, double* __restrict__ ce) {
-  const long long NN = g.NN;
-  const long long n = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (n >= NN) return;
-  if (!emask[n]) { ce[n] = 0.0; return; }
-  int iy, ix, iz;
-  node_yxz(n, g, iy, ix, iz);
-  double ue[24];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const long long nb = nidx(g, iy + kLocDy[j], ix + kLocDx[j], iz + kLocDz[j]);
-    ue[3 * j] = u[nb];
-    ue[3 * j + 1] = u[NN + nb];
-    ue[3 * j + 2] = u[2 * NN + nb];
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < 24; ++k) {
-    double t = 0.0;
-#pragma unroll
-    for (int b = 0; b < 24; ++b) t += ue[b] * Ke[k * 24 + b];
-    s += t * ue[k];
-  }
-  ce[n] = s;
-}
-
-void launch_compliance(hipStream_t s, const Geo& g, const double* Ke, const uint8_t* emask, const double* u,
-                       double* ce) {
-  hipLaunchKernelGGL(k_compliance, dim3(grid_for(g.NN)), dim3(kBlock), 0, s, g, Ke, emask, u, ce);
-}
-
-// =========================================================================
 // Jacobi diagonals (1001-1050).
 // =========================================================================
 __global__ void k_diag_fine(Geo g, const double* __restrict__ Ke, const double* __restrict__ E,
