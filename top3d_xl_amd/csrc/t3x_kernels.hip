@@ -667,12 +667,12 @@ __global__ __launch_bounds__(kBlock) void k_dot(const double* __restrict__ a, co
   (void)sq;
 }
 void launch_norm2(hipStream_t s, const double* v, long long n, const RedWs& ws, double* out) {
-  hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, s, v, v, n, ws, out, 1);
-  fin1(s, FIN_SQRT, red_blocks(n), ws, nullptr, nullptr, out);
+  hipLaunchKernelGGL(k_dot, dim3(red1_blocks(n)), dim3(kBlock), 0, s, v, v, n, ws, out, 1);
+  fin1(s, FIN_SQRT, red1_blocks(n), ws, nullptr, nullptr, out);
 }
 void launch_dot(hipStream_t s, const double* a, const double* b, long long n, const RedWs& ws, double* out) {
-  hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, s, a, b, n, ws, out, 0);
-  fin1(s, FIN_SUM, red_blocks(n), ws, nullptr, nullptr, out);
+  hipLaunchKernelGGL(k_dot, dim3(red1_blocks(n)), dim3(kBlock), 0, s, a, b, n, ws, out, 0);
+  fin1(s, FIN_SUM, red1_blocks(n), ws, nullptr, nullptr, out);
 }
 
 // y += lambda p ; r -= lambda q ; resnorm = ||r|| / ||b|| ; convergence flag.
@@ -793,8 +793,8 @@ __global__ __launch_bounds__(kBlock) void k_scale_diag(double* __restrict__ z, c
 void launch_scale_diag(hipStream_t s, double* z, const double* dinv, const double* r, long long n,
                        const RedWs* ws, double* sc, const int* fl) {
   const int mode = fl ? 2 : 1;
-  hipLaunchKernelGGL(k_scale_diag, dim3(red_blocks(n)), dim3(kBlock), 0, s, z, dinv, r, n, *ws, sc, fl, mode);
-  fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, red_blocks(n), *ws, sc, (int*)fl, nullptr);
+  hipLaunchKernelGGL(k_scale_diag, dim3(red1_blocks(n)), dim3(kBlock), 0, s, z, dinv, r, n, *ws, sc, fl, mode);
+  fin1(s, mode == 1 ? FIN_ZR_INIT : FIN_ZR_LOOP, red1_blocks(n), *ws, sc, (int*)fl, nullptr);
 }
 
 __global__ void k_gather_compact(const int* __restrict__ map, int n, int ncomp, long long NN,
@@ -1033,6 +1033,8 @@ __global__ __launch_bounds__(kBlock) void k_masked_sum(long long n, const uint8_
 }
 void launch_masked_sum(hipStream_t s, long long n, const uint8_t* emask, const double* a, const double* b, int mode,
                        const RedWs& ws, double* out) {
+  // red_blocks, not red1_blocks: the allocation-free OC entry point runs this
+  // sum with its partials in a fixed scratch sized for kMaxRedBlocks blocks
   hipLaunchKernelGGL(k_masked_sum, dim3(red_blocks(n)), dim3(kBlock), 0, s, n, emask, a, b, mode, ws, out);
   fin1(s, FIN_SUM, red_blocks(n), ws, nullptr, nullptr, out);
 }
