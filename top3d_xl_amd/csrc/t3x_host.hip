@@ -186,7 +186,9 @@ struct t3x_problem {
   double* dT2 = nullptr;
   uint8_t* fix = nullptr;
   uint8_t* act = nullptr;  // finest node chunks (16 nodes) with a solid node, or null when all are
-  Act fineAct() const { return Act{act, lv[0].NN}; }
+  int* actList = nullptr;  // ids of those chunks (ascending), nActList of them
+  int nActList = 0;
+  Act fineAct() const { return Act{act, lv[0].NN, actList, nActList}; }
   uint8_t* passive = nullptr;  // passive cells (passiveElements_), or null
   uint8_t* active = nullptr;   // solid and not passive (MMA design variables, 274-276)
   int nPassive = 0;
@@ -511,6 +513,7 @@ long long footprint_estimate(int nely, int nelx, int nelz, long long nv, const t
   add(8LL * std::max(kMaxRedBlocks * 8, kMaxPartials) + 4096 * 4);  // reductions, scalars, flags
   add(4LL * kMaxPartials);                           // K*u tile list (irregular domains)
   add(NN / 16 + 1);                                  // node-chunk flags (irregular domains)
+  add(4 * (NN / 16 + 1));                            // and their id list
   return b + 4096 * nalloc;
 }
 
@@ -639,6 +642,12 @@ int setup(t3x_problem& P, const uint8_t* vox, int nFix, const int* fixNodes, con
       if (!all) {
         P.act = P.alloc<uint8_t>(ch.size());
         P.upload(P.act, ch.data(), ch.size());
+        std::vector<int> ids;
+        for (size_t c = 0; c < ch.size(); ++c)
+          if (ch[c]) ids.push_back((int)c);
+        P.nActList = (int)ids.size();
+        P.actList = P.alloc<int>(ids.size());
+        if (!ids.empty()) P.upload(P.actList, ids.data(), ids.size() * sizeof(int));
       }
     }
     if (P.mixed()) {

@@ -31,15 +31,30 @@ inline Geo geo(const Level& L) {
 // for node n (16 nodes: one 128-byte line of a component in FP64), a nodal
 // 3-vector's element i standing for node i mod NN.  The vector kernels skip
 // void chunks (their entries are 0 and stay 0); chunk null: every chunk.
+// list: the same chunks as ascending ids (nList of them), so that the vector
+// kernels launch work only for them.
 struct Act {
   const uint8_t* chunk = nullptr;
   long long NN = 0;
+  const int* list = nullptr;
+  int nList = 0;
 };
 __device__ __forceinline__ bool act_skip(const Act& a, long long i) {
   if (!a.chunk) return false;
   if (i >= a.NN) i -= a.NN;
   if (i >= a.NN) i -= a.NN;
   return !a.chunk[i >> 4];
+}
+// A vector kernel over n elements (n = NN or 3 NN when a.list is set) runs
+// act_items work items; item g is node act_node(g) (at or past NN: none) and
+// covers its act_ncomp components, element node + c NN.  Without a list,
+// item g is element g.
+__host__ __device__ __forceinline__ long long act_items(const Act& a, long long n) {
+  return a.list ? 16LL * a.nList : n;
+}
+__host__ __device__ __forceinline__ int act_ncomp(const Act& a, long long n) { return a.list ? (int)(n / a.NN) : 1; }
+__device__ __forceinline__ long long act_node(const Act& a, long long g) {
+  return a.list ? 16LL * a.list[g >> 4] + (g & 15) : g;
 }
 
 // Reduction workspace: partials (kMaxRedBlocks * 8 doubles) + one counter.

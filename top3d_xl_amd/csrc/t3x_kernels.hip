@@ -687,14 +687,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(double* __restrict__ r, c
   // y += lambda p (616) is deferred to k_pcg_pupdate, which reads p anyway
   const double lambda = sc[SC_LAMBDA];
   double rr = 0.0;
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (act_skip(act, i)) continue;
-    const double ri = r[i] - lambda * q[i];     // 617
-    r[i] = ri;
-    rr += ri * ri;
-    if (xj) {
-      const double dd = (double)dj[i];
-      xj[i] = dd != 0.0 ? (float)(omega * (ri / dd)) : 0.f;
+  const long long m = act_items(act, n);
+  const int nc = act_ncomp(act, n);
+  for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < m; g += (long long)gridDim.x * kBlock) {
+    const long long b = act_node(act, g);
+    if (act.list && b >= act.NN) continue;
+    for (int c = 0; c < nc; ++c) {
+      const long long i = b + c * act.NN;
+      const double ri = r[i] - lambda * q[i];     // 617
+      r[i] = ri;
+      rr += ri * ri;
+      if (xj) {
+        const double dd = (double)dj[i];
+        xj[i] = dd != 0.0 ? (float)(omega * (ri / dd)) : 0.f;
+      }
     }
   }
   double v[1] = {rr};
@@ -704,9 +710,9 @@ void launch_pcg_update(hipStream_t s, double* y, double* r, const double* p, con
                        const RedWs& ws, double* sc, int* fl, const float* dj, float* xj, double omega, Act act) {
   (void)y;
   (void)p;
-  hipLaunchKernelGGL(k_pcg_update, dim3(red1_blocks(n)), dim3(kBlock), 0, s, r, q, n, ws, sc, fl, dj, xj, omega,
-                     act);
-  fin1(s, FIN_RR, red1_blocks(n), ws, sc, fl, nullptr);
+  const int nb = red1_blocks(act_items(act, n));
+  hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(kBlock), 0, s, r, q, n, ws, sc, fl, dj, xj, omega, act);
+  fin1(s, FIN_RR, nb, ws, sc, fl, nullptr);
 }
 
 // y += lambda p (616) for the iteration just finished, then p = z + beta p (630).
@@ -714,16 +720,23 @@ __global__ void k_pcg_pupdate(double* __restrict__ y, double* __restrict__ p, co
                               long long n, const double* sc, const int* fl, Act act) {
   if (fl[FL_DONE]) return;
   const double beta = sc[SC_BETA], lambda = sc[SC_LAMBDA];
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (act_skip(act, i)) continue;
-    const double pi = p[i];
-    y[i] = y[i] + lambda * pi;
-    p[i] = z[i] + beta * pi;
+  const long long m = act_items(act, n);
+  const int nc = act_ncomp(act, n);
+  for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < m; g += (long long)gridDim.x * kBlock) {
+    const long long b = act_node(act, g);
+    if (act.list && b >= act.NN) continue;
+    for (int c = 0; c < nc; ++c) {
+      const long long i = b + c * act.NN;
+      const double pi = p[i];
+      y[i] = y[i] + lambda * pi;
+      p[i] = z[i] + beta * pi;
+    }
   }
 }
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const double* z, long long n, const double* sc,
                         const int* fl, Act act) {
-  hipLaunchKernelGGL(k_pcg_pupdate, dim3(stream_blocks(n)), dim3(kBlock), 0, s, y, p, z, n, sc, fl, act);
+  hipLaunchKernelGGL(k_pcg_pupdate, dim3(stream_blocks(act_items(act, n))), dim3(kBlock), 0, s, y, p, z, n, sc, fl,
+                     act);
 }
 // After the loop: the converging iteration skipped its p-update, so its
 // y += lambda p is applied here (no-op if the solve hit maxIT).
@@ -731,12 +744,20 @@ __global__ void k_pcg_finish(double* __restrict__ y, const double* __restrict__ 
                              const int* fl, Act act) {
   if (!fl[FL_DONE]) return;
   const double lambda = sc[SC_LAMBDA];
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-    if (!act_skip(act, i)) y[i] = y[i] + lambda * p[i];
+  const long long m = act_items(act, n);
+  const int nc = act_ncomp(act, n);
+  for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < m; g += (long long)gridDim.x * kBlock) {
+    const long long b = act_node(act, g);
+    if (act.list && b >= act.NN) continue;
+    for (int c = 0; c < nc; ++c) {
+      const long long i = b + c * act.NN;
+      y[i] = y[i] + lambda * p[i];
+    }
+  }
 }
 void launch_pcg_finish(hipStream_t s, double* y, const double* p, long long n, const double* sc, const int* fl,
                        Act act) {
-  hipLaunchKernelGGL(k_pcg_finish, dim3(stream_blocks(n)), dim3(kBlock), 0, s, y, p, n, sc, fl, act);
+  hipLaunchKernelGGL(k_pcg_finish, dim3(stream_blocks(act_items(act, n))), dim3(kBlock), 0, s, y, p, n, sc, fl, act);
 }
 
 // Mixed precision: the V-cycle output z is FP32; y, r, p and scalars FP64.
@@ -744,16 +765,23 @@ __global__ void k_pcg_pupdate_mixed(double* __restrict__ y, double* __restrict__
                                     long long n, const double* sc, const int* fl, Act act) {
   if (fl[FL_DONE]) return;
   const double beta = sc[SC_BETA], lambda = sc[SC_LAMBDA];
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (act_skip(act, i)) continue;
-    const double pi = p[i];
-    y[i] = y[i] + lambda * pi;
-    p[i] = (double)z[i] + beta * pi;
+  const long long m = act_items(act, n);
+  const int nc = act_ncomp(act, n);
+  for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < m; g += (long long)gridDim.x * kBlock) {
+    const long long b = act_node(act, g);
+    if (act.list && b >= act.NN) continue;
+    for (int c = 0; c < nc; ++c) {
+      const long long i = b + c * act.NN;
+      const double pi = p[i];
+      y[i] = y[i] + lambda * pi;
+      p[i] = (double)z[i] + beta * pi;
+    }
   }
 }
 void launch_pcg_pupdate(hipStream_t s, double* y, double* p, const float* z, long long n, const double* sc,
                         const int* fl, Act act) {
-  hipLaunchKernelGGL(k_pcg_pupdate_mixed, dim3(stream_blocks(n)), dim3(kBlock), 0, s, y, p, z, n, sc, fl, act);
+  hipLaunchKernelGGL(k_pcg_pupdate_mixed, dim3(stream_blocks(act_items(act, n))), dim3(kBlock), 0, s, y, p, z, n, sc,
+                     fl, act);
 }
 template <typename TD, typename TS>
 __global__ void k_convert(TD* __restrict__ dst, const TS* __restrict__ src, long long n) {

@@ -396,10 +396,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi0(const TR* __restrict__ r, co
                                                     TO* __restrict__ x, double omega, long long n, const int* fl,
                                                     Act act) {
   if (fl && fl[FL_DONE]) return;
-  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
-    if (act_skip(act, i)) continue;
-    const double dd = (double)d[i];
-    x[i] = dd != 0.0 ? (TO)(omega * ((double)r[i] / dd)) : TO(0);
+  const long long m = act_items(act, n);
+  const int nc = act_ncomp(act, n);
+  for (long long g = (long long)blockIdx.x * kBlock + threadIdx.x; g < m; g += (long long)gridDim.x * kBlock) {
+    const long long b = act_node(act, g);
+    if (act.list && b >= act.NN) continue;
+    for (int c = 0; c < nc; ++c) {
+      const long long i = b + c * act.NN;
+      const double dd = (double)d[i];
+      x[i] = dd != 0.0 ? (TO)(omega * ((double)r[i] / dd)) : TO(0);
+    }
   }
 }
 
@@ -457,17 +463,17 @@ void launch_prolong_add(hipStream_t s, const Geo& gf, const Geo& gc, int span, c
 
 void launch_jacobi0(hipStream_t s, const double* r, const float* d, float* x, double omega, long long n,
                     const int* fl, Act act) {
-  hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, d, x, omega,
+  hipLaunchKernelGGL((k_jacobi0<double, float, float>), dim3(stream_blocks(act_items(act, n))), dim3(kBlock), 0, s, r, d, x, omega,
                      n, fl, act);
 }
 void launch_jacobi0(hipStream_t s, const double* r, const double* d, double* x, double omega, long long n,
                     const int* fl, Act act) {
-  hipLaunchKernelGGL((k_jacobi0<double, double, double>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, d, x,
+  hipLaunchKernelGGL((k_jacobi0<double, double, double>), dim3(stream_blocks(act_items(act, n))), dim3(kBlock), 0, s, r, d, x,
                      omega, n, fl, act);
 }
 void launch_jacobi0(hipStream_t s, const float* r, const float* d, float* x, double omega, long long n,
                     const int* fl, Act act) {
-  hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(stream_blocks(n)), dim3(kBlock), 0, s, r, d, x, omega,
+  hipLaunchKernelGGL((k_jacobi0<float, float, float>), dim3(stream_blocks(act_items(act, n))), dim3(kBlock), 0, s, r, d, x, omega,
                      n, fl, act);
 }
 
