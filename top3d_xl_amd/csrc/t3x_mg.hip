@@ -173,14 +173,16 @@ __global__ __launch_bounds__(kBlock) void k_restrict_xz(Geo gf, Geo gc, const TM
 
 // out = omega*(r/d) + P xc + (omega*r)/d on solid nodes, 0 elsewhere (758,
 // 770, 773); optional fused z.r partials (629).  Block (bx, bz) covers 256
-// (y, x) columns and the node planes [bz*zc, bz*zc + zc); every load is in
-// bounds (zero weights read a clamped neighbour), so the loads of a plane
-// issue together and the mask only selects the result.
+// (y, x) columns and the node planes [bz*zc, bz*zc + zc); a thread walks its
+// column plane by plane, reloading the 8 coarse values (clamped neighbours
+// carry weight 0) only when a solid node needs another coarse z interval;
+// void nodes are neither loaded nor stored.
 template <int S, typename TX, typename TR, typename TD, typename TO>
-__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(Geo gf, Geo gc, double omega, int zc, const TX* __restrict__ xc,
-                                                    const TR* __restrict__ rf, const TD* __restrict__ diag,
-                                                    const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
-                                                    double* __restrict__ partials, const int* fl) {
+__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(Geo gf, Geo gc, double omega, int zc,
+                                                           const TX* __restrict__ xc, const TR* __restrict__ rf,
+                                                           const TD* __restrict__ diag,
+                                                           const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
+                                                           double* __restrict__ partials, const int* fl) {
   if (fl && fl[FL_DONE]) return;
   const int plane = gf.NY * gf.NX;
   const long long NN = gf.NN, NC = gc.NN;
