@@ -376,7 +376,7 @@ def test_mma_matches_independent_subproblem_solve():
 
 
 def test_mma_converges_to_analytic_optimum():
-    """Sequential MMA with the driver's move limit 0.1 (356, 364-365) on
+    """Sequential MMA with the LOCAL loop's move limit 0.1 (356, 364-365) on
     min sum w_j / x_j  s.t.  mean(x) <= V: the optimum is x_j proportional
     to sqrt(w_j) (Lagrange), reached to 1e-6."""
     rng = np.random.default_rng(41)

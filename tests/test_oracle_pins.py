@@ -2,7 +2,7 @@
 constructions: Gauss quadrature, hat-function prolongation, scipy assembly,
 dense linear algebra and numpy restatements of the loop algebra.
 
-The reference ships no tests or golden vectors (SURVEY: none exist), so these
+The reference ships no tests or golden vectors, so these
 checks are what make the oracle trustworthy.  Each one is chosen so a
 plausible restatement mistake (wrong local-node offset, flipped y axis, a
 missing BC term, a wrong interpolation weight, a transposed block) fails it.
