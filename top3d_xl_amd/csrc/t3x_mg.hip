@@ -260,7 +260,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_jacobi(Geo gf, Geo gc, doubl
 // FP64 instance needs 212 VGPRs and ran 531 against 461 us).
 constexpr int kPZ = 4;
 template <int S, typename TX, typename TO>
-__global__ __launch_bounds__(kBlock) void k_prolong_add(Geo gf, Geo gc, int zc, const TX* __restrict__ xc,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(S == 4 && sizeof(TO) == 4 ? 4 : 1)))
+void k_prolong_add(Geo gf, Geo gc, int zc, const TX* __restrict__ xc,
                                                         const uint8_t* __restrict__ nmaskF, TO* __restrict__ out,
                                                         const int* fl) {
   static_assert(kPZ % S == 0, "a plane group spans whole coarse intervals");
