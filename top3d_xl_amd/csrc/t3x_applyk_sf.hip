@@ -146,7 +146,10 @@ enum { SEPI_KU = 0, SEPI_RESID = 1, SEPI_SMOOTH = 2 };
 //   SEPI_SMOOTH y = u + omega (r - Ku) / d     partials: y.r (post-smoothing, 775)
 // T: arithmetic/LDS type; TE, TU, TY, TR, TD: types of E, u, y, r, diag in HBM.
 template <typename T, typename TE, typename TU, typename TY, int EPI, typename TR, typename TD>
-__global__ __launch_bounds__(SF_T) void k_apply_K_sf(Geo g, int zc, SfCoef cf, const TE* __restrict__ E,
+// (the FP32 residual instance bounded to five waves per SIMD: 95 VGPRs
+// without scratch; the smoothing instance would spill)
+__global__ __launch_bounds__(SF_T) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && EPI == SEPI_RESID ? 5 : 1)))
+void k_apply_K_sf(Geo g, int zc, SfCoef cf, const TE* __restrict__ E,
                                                      const uint8_t* __restrict__ fix, const TU* __restrict__ u,
                                                      TY* __restrict__ y, double* __restrict__ partials,
                                                      const int* fl, int reduce, const TR* __restrict__ rv,
